@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit DG Euler RK stage on one MI355X.
+
+A step is one SSP-RK3 time step (3 stages; each stage = residual + inverse mass +
+stage update) over the whole mesh, with the state resident in HBM.  The metric
+is DOF-updates per second per RK stage: 5*N*E*3*K / t over K timed steps.
+
+Default workload: the 196,608-tet periodic cube (32^3 hexes x 6 tets, [0,10]^3)
+at P3 (N = 20 modes x 5 variables) with the Roe flux in FP64, isentropic vortex
+(gamma 1.4, strength 5, centre (5,5), free stream (1,1,0)) as the state.
+
+Printed: ONE JSON line with the metric, the roofline of the dominant kernel
+(measured with HIP events on the solver stream), and the CPU oracle timed on
+a bounded sample on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM = 8.0e12      # B/s
+PEAK_FP64 = 78.6e12    # flop/s (MI355X dense FP64, vector = matrix)
+
+WORKLOADS = {
+    # name: (hexes per axis, order, flux, jitter)
+    "vortex_p3_196k": (32, 3, "roe", 0.0),
+    "vortex_p3_1296k": (60, 3, "roe", 0.1),
+    "vortex_p1_3k": (8, 1, "lf", 0.0),
+}
+VORTEX = [1.4, 5.0, 5.0, 5.0, 1.0, 1.0, 10.0, 10.0]
+
+
+def algorithmic(order, Q, Qf, N):
+    """Per element and stage: flops of the volume and surface kernels, bytes of the update kernel."""
+    vol_flops = 40 * Q * N + 145 * Q            # interpolation 10QN, test 30QN, pointwise flux ~145Q
+    surf_flops = 80 * Qf * N + 410 * Qf         # 2 faces: traces 20QfN, test 20QfN, Riemann ~205Qf each
+    upd_bytes = 4 * 8 * 5 * N + 8               # read U^n, U^s, R; write U; |det|
+    stage_bytes = 8 * 5 * N * (2 + 2 / 3) + 168  # compulsory traffic of one fused stage
+    return vol_flops, surf_flops, upd_bytes, stage_bytes
+
+
+def cpu_baseline(workload_n, order, flux, seconds_target=15.0):
+    """The oracle (C restatement, OpenMP, reference loop structure) on a bounded
+    sample of the same workload: a 16^3 x 6 = 24,576-tet periodic box at the
+    same order and flux, one RK stage (residual + inverse mass + update)."""
+    import numpy as np
+
+    import fve_dg_amd as F
+    from oracle.binding import Oracle
+
+    O = Oracle()
+    threads = O.max_threads()
+    n = 16 if workload_n >= 16 else workload_n
+    mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7)
+    m = mesh.export()
+    U = mesh.project(order, "vortex", VORTEX)
+    M = O.mass(m, order)
+    E = len(m["cells"])
+    t0 = time.perf_counter()
+    R = O.residual(m, order, U, flux=flux, threads=threads)
+    _ = U + 0.001 * (-R / M)
+    t = time.perf_counter() - t0
+    dof = 5 * F.num_basis(order) * E
+    return {"value": dof / t, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": f"one RK stage on a {n}^3x6 = {E}-tet periodic box, P{order} {flux}, "
+                      f"oracle/dg_oracle.c with {threads} OpenMP threads ({t:.2f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="vortex_p3_196k", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the multi-GPU path is a design (element partition + halo "
+              "exchange), not built; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import fve_dg_amd as F
+
+    n, order, flux, jitter = WORKLOADS[args.workload]
+    t_setup = time.perf_counter()
+    mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7, jitter=jitter, seed=60)
+    U = mesh.project(order, "vortex", VORTEX)
+    E = mesh.n_cells
+    N = F.num_basis(order)
+    vp, vw, fp, fw = F.quadrature(order)
+    Q, Qf = len(vw), len(fw)
+    s = F.Solver(0)
+    s.set_scheme(order, flux=flux).set_mesh(mesh)
+    s.upload(U)
+    h = mesh.geometry()["h"].min() * 3.0
+    dt = 0.3 * h / ((2 * order + 1) * s.max_wave_speed())
+    setup_s = time.perf_counter() - t_setup
+    print(f"[bench] {args.workload}: E={E} P{order} {flux} dt={dt:.3e} setup {setup_s:.1f}s", file=sys.stderr,
+          flush=True)
+
+    if args.warmup > 0:
+        s.ssprk3(dt, args.warmup)
+    t0 = time.perf_counter()
+    ms = s.time_stages(dt, 3 * args.steps)   # ms[0]: per-stage average over the timed region
+    wall = time.perf_counter() - t0
+    stage_ms = ms[0]
+    dof = 5 * N * E
+    value = dof / (stage_ms * 1e-3)
+
+    vol_f, surf_f, upd_b, stage_b = algorithmic(order, Q, Qf, N)
+    kernels = {
+        "volume": {"ms": ms[1], "achieved": vol_f * E / (ms[1] * 1e-3), "peak": PEAK_FP64, "bound": "mfma",
+                   "unit": "TFLOP/s", "scale": 1e12},
+        "surface": {"ms": ms[2], "achieved": surf_f * E / (ms[2] * 1e-3), "peak": PEAK_FP64, "bound": "mfma",
+                    "unit": "TFLOP/s", "scale": 1e12},
+        "update": {"ms": ms[3], "achieved": upd_b * E / (ms[3] * 1e-3), "peak": PEAK_HBM, "bound": "hbm",
+                   "unit": "GB/s", "scale": 1e9},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    k = kernels[dom]
+    roofline = {"bound": k["bound"], "achieved": round(k["achieved"] / k["scale"], 3),
+                "peak": round(k["peak"] / k["scale"], 1), "unit": k["unit"],
+                "frac": round(k["achieved"] / k["peak"], 4), "traffic": None,
+                "kernel": dom, "kernel_ms": round(k["ms"], 4),
+                "all_kernels": {name: {"ms": round(v["ms"], 4), "achieved": round(v["achieved"] / v["scale"], 3),
+                                       "unit": v["unit"], "frac": round(v["achieved"] / v["peak"], 4)}
+                                for name, v in kernels.items()}}
+    out = {
+        "metric": "DOF-updates/s per RK stage, P3 tet Euler (FP64)",
+        "value": round(value, 1),
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(3 * stage_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (isentropic vortex projected on the mesh)",
+        "config": {"workload": args.workload, "tets": E, "order": order, "modes": N, "flux": flux,
+                   "dofs": dof, "stages_per_step": 3, "precision": "fp64", "wall_s": round(wall, 3)},
+        "roofline": roofline,
+    }
+    Ug = s.download()
+    if not np.isfinite(Ug).all():
+        print("[bench] non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(3)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, order, flux)
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

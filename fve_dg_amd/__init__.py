@@ -1,0 +1,287 @@
+"""Python binding of libfvdg.so, the MI355X explicit DG Euler solver.
+
+Everything goes through the C ABI declared in ``include/fvdg.h`` (ctypes, numpy
+arrays for the buffers).  The library must have been built
+(``python -m fve_dg_amd.build``); there is no fallback: a missing library or a
+failing call raises :class:`FvdgError`.
+
+State arrays use the reference's LongVector<5N> layout: shape (E, 5N), entry
+``[e, 5k+v]`` for mode k and variable v in (rho, rho u, rho v, rho w, rho E).
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libfvdg.so"
+_lib = None
+
+FLUX = {"lf": 0, "roe": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+FIELD = {"vortex": 0, "tgv": 1, "const": 2}
+PART_CELLS, PART_INTERNALS, PART_BOUNDARYS, PART_ALL = 1, 2, 4, 7
+BC_DIRICHLET, BC_PSEUDO3DZ = 0, 5
+
+_d = C.POINTER(C.c_double)
+_i64 = C.POINTER(C.c_int64)
+_u8 = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+
+class FvdgError(RuntimeError):
+    """A libfvdg call returned a nonzero status."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"libfvdg status {status}: {msg}")
+        self.status = status
+
+
+def _sig(lib, name, args, res=C.c_int):
+    fn = getattr(lib, name)
+    fn.argtypes = args
+    fn.restype = res
+
+
+def lib():
+    """Load libfvdg.so (once)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise FvdgError(-1, f"{_LIB_PATH} is not built; run `python -m fve_dg_amd.build`")
+        L = C.CDLL(str(_LIB_PATH))
+        _sig(L, "fvdg_version", [], C.c_char_p)
+        _sig(L, "fvdg_last_error", [], C.c_char_p)
+        _sig(L, "fvdg_num_basis", [C.c_int])
+        _sig(L, "fvdg_basis_eval", [C.c_int, C.c_double, C.c_double, C.c_double, _d, _d])
+        _sig(L, "fvdg_quadrature", [C.c_int, C.POINTER(C.c_int), _d, _d, C.POINTER(C.c_int), _d, _d])
+        _sig(L, "fvdg_mesh_create_box", [C.POINTER(_vp), _d, _d, _i64, C.c_int, C.c_int, C.c_double, C.c_uint64])
+        _sig(L, "fvdg_mesh_create", [C.POINTER(_vp), C.c_int64, _d, C.c_int64, _i64, C.c_int64, _i64, _i64, _i64])
+        _sig(L, "fvdg_mesh_sizes", [_vp, _i64, _i64, _i64])
+        _sig(L, "fvdg_mesh_export", [_vp, _d, _i64, _i64, _i64, _i64])
+        _sig(L, "fvdg_mesh_geometry", [_vp, _d, _d, _d, _d, _d])
+        _sig(L, "fvdg_mesh_set_boundary_types", [_vp, _u8])
+        _sig(L, "fvdg_mesh_get_boundary_types", [_vp, _u8])
+        _sig(L, "fvdg_mesh_tag_boundaries_pseudo3dz", [_vp])
+        _sig(L, "fvdg_mesh_destroy", [_vp], None)
+        _sig(L, "fvdg_project_analytic", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d])
+        _sig(L, "fvdg_analytic_error", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d, _d])
+        _sig(L, "fvdg_create", [C.POINTER(_vp), C.c_int])
+        _sig(L, "fvdg_destroy", [_vp], None)
+        _sig(L, "fvdg_set_scheme", [_vp, C.c_int, C.c_int, C.c_double, C.c_int])
+        _sig(L, "fvdg_set_mesh", [_vp, _vp])
+        _sig(L, "fvdg_state_upload", [_vp, _d])
+        _sig(L, "fvdg_state_download", [_vp, _d])
+        _sig(L, "fvdg_dirichlet_points", [_vp, _i64, _i64, _d])
+        _sig(L, "fvdg_set_dirichlet_states", [_vp, _d])
+        _sig(L, "fvdg_residual", [_vp, C.c_double, C.c_int, _d])
+        _sig(L, "fvdg_mass_diagonal", [_vp, _d])
+        _sig(L, "fvdg_ssprk3_step", [_vp, C.c_double, C.c_int])
+        _sig(L, "fvdg_max_wave_speed", [_vp, _d])
+        _sig(L, "fvdg_conservation_sums", [_vp, _d])
+        _sig(L, "fvdg_time_stages", [_vp, C.c_double, C.c_int, _d])
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise FvdgError(status, lib().fvdg_last_error().decode())
+
+
+def _p(a, ctype=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ctype))
+
+
+def num_basis(order):
+    return (order + 1) * (order + 2) * (order + 3) // 6
+
+
+def basis(order, x, y, z):
+    """(values (N,), reference gradients (N,3)) of the modal basis at (x,y,z)."""
+    n = num_basis(order)
+    v, g = np.zeros(n), np.zeros((n, 3))
+    st = lib().fvdg_basis_eval(order, x, y, z, _p(v), _p(g))
+    if st < 0:
+        _check(st)
+    return v, g
+
+
+def quadrature(order):
+    """(volume points (Q,3), weights (Q,), face points (Qf,2), weights (Qf,)) of the order's rules."""
+    nq, nf = C.c_int(), C.c_int()
+    _check(lib().fvdg_quadrature(order, C.byref(nq), None, None, C.byref(nf), None, None))
+    vp, vw = np.zeros((nq.value, 3)), np.zeros(nq.value)
+    fp, fw = np.zeros((nf.value, 2)), np.zeros(nf.value)
+    _check(lib().fvdg_quadrature(order, None, _p(vp), _p(vw), None, _p(fp), _p(fw)))
+    return vp, vw, fp, fw
+
+
+class Mesh:
+    """Host tetrahedral mesh (ComputingMesh) owned by libfvdg."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def box(cls, n, lb=(0.0, 0.0, 0.0), ub=(1.0, 1.0, 1.0), split=6, periodic=0, jitter=0.0, seed=0):
+        n = (n, n, n) if np.isscalar(n) else tuple(n)
+        h = _vp()
+        lb_, ub_ = np.asarray(lb, dtype=np.float64), np.asarray(ub, dtype=np.float64)
+        n_ = np.asarray(n, dtype=np.int64)
+        _check(lib().fvdg_mesh_create_box(C.byref(h), _p(lb_), _p(ub_), _p(n_, C.c_int64), split, periodic,
+                                          jitter, seed))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, points, cells, face_nodes, face_cells, face_nodes_r=None):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        cl = np.ascontiguousarray(cells, dtype=np.int64)
+        fn = np.ascontiguousarray(face_nodes, dtype=np.int64)
+        fc = np.ascontiguousarray(face_cells, dtype=np.int64)
+        fr = None if face_nodes_r is None else np.ascontiguousarray(face_nodes_r, dtype=np.int64)
+        h = _vp()
+        _check(lib().fvdg_mesh_create(C.byref(h), len(pts), _p(pts), len(cl), _p(cl, C.c_int64), len(fn),
+                                      _p(fn, C.c_int64), _p(fc, C.c_int64),
+                                      None if fr is None else _p(fr, C.c_int64)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fvdg_mesh_destroy(self._h)
+            self._h = None
+
+    @property
+    def sizes(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().fvdg_mesh_sizes(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    @property
+    def n_cells(self):
+        return self.sizes[1]
+
+    def export(self):
+        npnt, nc, nf = self.sizes
+        out = dict(points=np.zeros((npnt, 3)), cells=np.zeros((nc, 4), np.int64),
+                   face_nodes=np.zeros((nf, 3), np.int64), face_cells=np.zeros((nf, 2), np.int64),
+                   face_nodes_r=np.zeros((nf, 3), np.int64))
+        _check(lib().fvdg_mesh_export(self._h, _p(out["points"]), _p(out["cells"], C.c_int64),
+                                      _p(out["face_nodes"], C.c_int64), _p(out["face_cells"], C.c_int64),
+                                      _p(out["face_nodes_r"], C.c_int64)))
+        return out
+
+    def geometry(self):
+        _, nc, nf = self.sizes
+        g = dict(inv_jac=np.zeros((nc, 3, 3)), det=np.zeros(nc), h=np.zeros(nc),
+                 normal=np.zeros((nf, 3)), area=np.zeros(nf))
+        _check(lib().fvdg_mesh_geometry(self._h, _p(g["inv_jac"]), _p(g["det"]), _p(g["h"]), _p(g["normal"]),
+                                        _p(g["area"])))
+        return g
+
+    def boundary_types(self):
+        bc = np.zeros(self.sizes[2], np.uint8)
+        _check(lib().fvdg_mesh_get_boundary_types(self._h, _p(bc, C.c_uint8)))
+        return bc
+
+    def set_boundary_types(self, bc):
+        bc = np.ascontiguousarray(bc, dtype=np.uint8)
+        _check(lib().fvdg_mesh_set_boundary_types(self._h, _p(bc, C.c_uint8)))
+
+    def tag_pseudo3dz(self):
+        _check(lib().fvdg_mesh_tag_boundaries_pseudo3dz(self._h))
+
+    def project(self, order, field, params, t=0.0):
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        U = np.zeros((self.n_cells, 5 * num_basis(order)))
+        _check(lib().fvdg_project_analytic(self._h, order, FIELD[field], _p(prm), len(prm), t, _p(U)))
+        return U
+
+    def analytic_error(self, order, field, params, t, U):
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        err = np.zeros(5)
+        _check(lib().fvdg_analytic_error(self._h, order, FIELD[field], _p(prm), len(prm), t, _p(U), _p(err)))
+        return err
+
+
+class Solver:
+    """One GPU solver context: scheme + mesh + device state."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        _check(lib().fvdg_create(C.byref(h), device))
+        self._h = h
+        self.order = None
+        self.n_cells = None
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fvdg_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_scheme(self, order, flux="lf", gamma=1.4, precision="fp64"):
+        _check(lib().fvdg_set_scheme(self._h, order, FLUX[flux], gamma, PRECISION[precision]))
+        self.order = order
+        return self
+
+    def set_mesh(self, mesh):
+        _check(lib().fvdg_set_mesh(self._h, mesh._h))
+        self.n_cells = mesh.n_cells
+        return self
+
+    @property
+    def nv(self):
+        return 5 * num_basis(self.order)
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != (self.n_cells, self.nv):
+            raise ValueError(f"state shape {U.shape} != {(self.n_cells, self.nv)}")
+        _check(lib().fvdg_state_upload(self._h, _p(U)))
+
+    def download(self):
+        U = np.zeros((self.n_cells, self.nv))
+        _check(lib().fvdg_state_download(self._h, _p(U)))
+        return U
+
+    def residual(self, t=0.0, parts=PART_ALL):
+        R = np.zeros((self.n_cells, self.nv))
+        _check(lib().fvdg_residual(self._h, t, parts, _p(R)))
+        return R
+
+    def mass(self):
+        M = np.zeros((self.n_cells, self.nv))
+        _check(lib().fvdg_mass_diagonal(self._h, _p(M)))
+        return M
+
+    def dirichlet_points(self):
+        n = C.c_int64()
+        _check(lib().fvdg_dirichlet_points(self._h, C.byref(n), None, None))
+        qf = len(quadrature(self.order)[3])
+        ids, xyz = np.zeros(n.value, np.int64), np.zeros((n.value, qf, 3))
+        _check(lib().fvdg_dirichlet_points(self._h, C.byref(n), _p(ids, C.c_int64), _p(xyz)))
+        return ids, xyz
+
+    def set_dirichlet_states(self, UD):
+        UD = np.ascontiguousarray(UD, dtype=np.float64)
+        _check(lib().fvdg_set_dirichlet_states(self._h, _p(UD)))
+
+    def ssprk3(self, dt, n_steps=1):
+        _check(lib().fvdg_ssprk3_step(self._h, dt, n_steps))
+
+    def max_wave_speed(self):
+        out = C.c_double()
+        _check(lib().fvdg_max_wave_speed(self._h, C.byref(out)))
+        return out.value
+
+    def conservation_sums(self):
+        out = np.zeros(5)
+        _check(lib().fvdg_conservation_sums(self._h, _p(out)))
+        return out
+
+    def time_stages(self, dt, n_stages):
+        ms = np.zeros(4)
+        _check(lib().fvdg_time_stages(self._h, dt, n_stages, _p(ms)))
+        return ms

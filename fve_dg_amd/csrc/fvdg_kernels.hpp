@@ -1,0 +1,353 @@
+// gfx950 kernels of the explicit DG Euler residual and SSP-RK3 update (FP64).
+//
+// Residual R(U) = R_cells + R_internals + R_boundarys, as in
+// src/DG/DG_Schemes/ExplicitConvection.cpp:
+//   R_cells  [e,j,v] = - sum_q w_q |J_e| (F(U_e(xi_q)) . (A_e^{-1} grad phi_j(xi_q)))_v   (:144-198)
+//   R_faces  [L,j,v] += sum_g w_g 2A_f Fhat_v(U_L,U_R,n) phi_j(xi_L(g))                 (:201-272)
+//            [R,j,v] -= sum_g w_g 2A_f Fhat_v(U_L,U_R,n) phi_j(xi_R(g))
+//   boundary faces use a ghost state: Dirichlet 2 U_D - U_in, Pseudo3DZ (1,1,1,-1,1) U_in  (:21-128)
+// These kernels are the plain FP64 path: pointwise fluxes go through scratch
+// buffers (Fq: E*Q*15, Fh: F*Qf*5) and every contraction is a thread-private
+// loop, so each output has one fixed summation order (deterministic, no atomics).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fvdg.h"
+
+struct KernelArgs {
+    int64_t E = 0, F = 0;
+    int N = 0, Q = 0, Qf = 0;
+    double gamma = 1.4;
+    int flux = 0, parts = 7;
+    const double* U = nullptr;
+    double* R = nullptr;
+    const double* cgeo = nullptr;     // E*10: A^{-1} row-major, |det A|
+    const int32_t* cfaces = nullptr;  // E*4: face*2 + side
+    const int32_t* fcell = nullptr;   // F*2
+    const uint8_t* fcombo = nullptr;  // F*2
+    const double* fgeo = nullptr;     // F*4: n, 2*area
+    const uint8_t* fbc = nullptr;     // F
+    const int32_t* fdir = nullptr;    // F
+    const double* UD = nullptr;       // Dirichlet states
+    const double* V = nullptr;        // Q*N
+    const double* G = nullptr;        // Q*3*N
+    const double* wq = nullptr;       // Q
+    const double* Phi = nullptr;      // 24*Qf*N
+    const double* wf = nullptr;       // Qf
+    double* Fq = nullptr;             // E*Q*15
+    double* Fh = nullptr;             // F*Qf*5
+};
+
+// element e, coefficient kv (= 5k+v) in the 64-element tiled layout
+__device__ __forceinline__ int64_t cidx(int64_t e, int kv, int NV) {
+    return ((e >> 6) * NV + kv) * 64 + (e & 63);
+}
+
+// ------------------------------------------------------------------ Euler algebra
+// Same formulas as src/DG/DG_Flux/EulerPhysicalFlux.cpp:7-18,46-55,70-89.
+__device__ __forceinline__ double pos_eps(double x) { return fmax(x, 1e-16); }
+
+__device__ __forceinline__ double euler_pressure(const double U[5], double gamma) {
+    const double rho = pos_eps(U[0]);
+    const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, e = U[4] / rho;
+    return (gamma - 1) * (rho * e - 0.5 * rho * (u * u + v * v + w * w));
+}
+
+__device__ __forceinline__ void euler_flux(const double U[5], double gamma, double F[5][3]) {
+    const double rho = pos_eps(U[0]);
+    const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, e = U[4] / rho;
+    const double p = euler_pressure(U, gamma);
+    F[0][0] = rho * u;         F[0][1] = rho * v;         F[0][2] = rho * w;
+    F[1][0] = rho * u * u + p; F[1][1] = rho * v * u;     F[1][2] = rho * w * u;
+    F[2][0] = rho * u * v;     F[2][1] = rho * v * v + p; F[2][2] = rho * w * v;
+    F[3][0] = rho * u * w;     F[3][1] = rho * v * w;     F[3][2] = rho * w * w + p;
+    F[4][0] = u * (rho * e + p); F[4][1] = v * (rho * e + p); F[4][2] = w * (rho * e + p);
+}
+
+__device__ __forceinline__ double euler_wave_speed(const double U[5], double gamma) {
+    const double rho = pos_eps(U[0]);
+    const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho;
+    const double c = sqrt(gamma * pos_eps(euler_pressure(U, gamma) / pos_eps(U[0])));
+    return sqrt(u * u + v * v + w * w) + c;
+}
+
+// Lax-Friedrichs: 0.5 (F(U_L) n + F(U_R) n + lambda (U_L - U_R)), lambda = max(|u|+c)
+__device__ __forceinline__ void flux_lf(const double UL[5], const double UR[5], const double n[3], double gamma,
+                                        double out[5]) {
+    double FL[5][3], FR[5][3];
+    euler_flux(UL, gamma, FL);
+    euler_flux(UR, gamma, FR);
+    const double lam = fmax(euler_wave_speed(UL, gamma), euler_wave_speed(UR, gamma));
+    for (int v = 0; v < 5; ++v) {
+        const double fl = FL[v][0] * n[0] + FL[v][1] * n[1] + FL[v][2] * n[2];
+        const double fr = FR[v][0] * n[0] + FR[v][1] * n[1] + FR[v][2] * n[2];
+        out[v] = 0.5 * (fl + fr + lam * (UL[v] - UR[v]));
+    }
+}
+
+// Roe (1981) flux with Harten's entropy fix on the acoustic waves, delta = 0.1 * a_roe:
+//   Fhat = 0.5 (F_L n + F_R n) - 0.5 sum_k |lambda_k| alpha_k r_k
+__device__ __forceinline__ void flux_roe(const double UL[5], const double UR[5], const double n[3], double gamma,
+                                         double out[5]) {
+    double FL[5][3], FR[5][3];
+    euler_flux(UL, gamma, FL);
+    euler_flux(UR, gamma, FR);
+    const double rL = pos_eps(UL[0]), rR = pos_eps(UR[0]);
+    const double pL = euler_pressure(UL, gamma), pR = euler_pressure(UR, gamma);
+    const double uL[3] = {UL[1] / rL, UL[2] / rL, UL[3] / rL};
+    const double uR[3] = {UR[1] / rR, UR[2] / rR, UR[3] / rR};
+    const double HL = (UL[4] + pL) / rL, HR = (UR[4] + pR) / rR;
+    const double sL = sqrt(rL), sR = sqrt(rR), inv = 1.0 / (sL + sR);
+    double ut[3], q2 = 0, qn = 0;
+    for (int d = 0; d < 3; ++d) {
+        ut[d] = (sL * uL[d] + sR * uR[d]) * inv;
+        q2 += ut[d] * ut[d];
+        qn += ut[d] * n[d];
+    }
+    const double Ht = (sL * HL + sR * HR) * inv;
+    const double a2 = fmax((gamma - 1) * (Ht - 0.5 * q2), 1e-16);
+    const double a = sqrt(a2);
+    const double rt = sL * sR;
+    const double dr = rR - rL, dp = pR - pL;
+    double du[3], dqn = 0;
+    for (int d = 0; d < 3; ++d) {
+        du[d] = uR[d] - uL[d];
+        dqn += du[d] * n[d];
+    }
+    double dut[3], ut_dut = 0;
+    for (int d = 0; d < 3; ++d) {
+        dut[d] = du[d] - dqn * n[d];
+        ut_dut += ut[d] * dut[d];
+    }
+    const double a1 = (dp - rt * a * dqn) / (2 * a2);
+    const double a5 = (dp + rt * a * dqn) / (2 * a2);
+    const double a_2 = dr - dp / a2;
+    const double delta = 0.1 * a;
+    auto fix = [delta](double lam) {
+        const double al = fabs(lam);
+        return al < delta ? (lam * lam + delta * delta) / (2 * delta) : al;
+    };
+    const double l1 = fix(qn - a), l2 = fabs(qn), l5 = fix(qn + a);
+    double d[5];
+    d[0] = l1 * a1 + l2 * a_2 + l5 * a5;
+    for (int k = 0; k < 3; ++k)
+        d[1 + k] = l1 * a1 * (ut[k] - a * n[k]) + l2 * a_2 * ut[k] + l2 * rt * dut[k] + l5 * a5 * (ut[k] + a * n[k]);
+    d[4] = l1 * a1 * (Ht - a * qn) + l2 * a_2 * 0.5 * q2 + l2 * rt * ut_dut + l5 * a5 * (Ht + a * qn);
+    for (int v = 0; v < 5; ++v) {
+        const double fl = FL[v][0] * n[0] + FL[v][1] * n[1] + FL[v][2] * n[2];
+        const double fr = FR[v][0] * n[0] + FR[v][1] * n[1] + FR[v][2] * n[2];
+        out[v] = 0.5 * (fl + fr) - 0.5 * d[v];
+    }
+}
+
+__device__ __forceinline__ void num_flux(int kind, const double UL[5], const double UR[5], const double n[3],
+                                         double gamma, double out[5]) {
+    if (kind == FVDG_FLUX_ROE)
+        flux_roe(UL, UR, n, gamma, out);
+    else
+        flux_lf(UL, UR, n, gamma, out);
+}
+
+// ------------------------------------------------------------------ layout transposes
+__global__ void k_aos_to_tiled(const double* __restrict__ aos, double* __restrict__ tiled, int64_t E, int NV) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E * NV) return;
+    const int64_t e = i / NV;
+    const int kv = (int)(i - e * NV);
+    tiled[cidx(e, kv, NV)] = aos[i];
+}
+__global__ void k_tiled_to_aos(const double* __restrict__ tiled, double* __restrict__ aos, int64_t E, int NV) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E * NV) return;
+    const int64_t e = i / NV;
+    const int kv = (int)(i - e * NV);
+    aos[i] = tiled[cidx(e, kv, NV)];
+}
+
+// ------------------------------------------------------------------ volume term
+// one thread per (element, volume point): trace of U and its physical flux
+__global__ void k_vol_flux(KernelArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.E * a.Q) return;
+    const int64_t e = t / a.Q;
+    const int q = (int)(t - e * a.Q);
+    const int NV = 5 * a.N;
+    double U[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < a.N; ++k) {
+        const double phi = a.V[q * a.N + k];
+        for (int v = 0; v < 5; ++v) U[v] += phi * a.U[cidx(e, 5 * k + v, NV)];
+    }
+    double F[5][3];
+    euler_flux(U, a.gamma, F);
+    double* out = a.Fq + t * 15;
+    for (int v = 0; v < 5; ++v)
+        for (int d = 0; d < 3; ++d) out[v * 3 + d] = F[v][d];
+}
+
+// one thread per (element, test function j): R = -sum_q (F . A^{-1} grad phi_j) w_q |J|
+__global__ void k_vol_test(KernelArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.E * a.N) return;
+    const int64_t e = t / a.N;
+    const int j = (int)(t - e * a.N);
+    const int NV = 5 * a.N;
+    const double* geo = a.cgeo + e * 10;
+    double Ai[9];
+    for (int i = 0; i < 9; ++i) Ai[i] = geo[i];
+    const double det = geo[9];
+    double r[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < a.Q; ++q) {
+        const double g0 = a.G[(q * 3 + 0) * a.N + j], g1 = a.G[(q * 3 + 1) * a.N + j], g2 = a.G[(q * 3 + 2) * a.N + j];
+        double gp[3];
+        for (int d = 0; d < 3; ++d) gp[d] = Ai[3 * d] * g0 + Ai[3 * d + 1] * g1 + Ai[3 * d + 2] * g2;
+        const double jw = a.wq[q] * det;
+        const double* F = a.Fq + (e * a.Q + q) * 15;
+        for (int v = 0; v < 5; ++v) r[v] -= (F[3 * v] * gp[0] + F[3 * v + 1] * gp[1] + F[3 * v + 2] * gp[2]) * jw;
+    }
+    for (int v = 0; v < 5; ++v) a.R[cidx(e, 5 * j + v, NV)] = r[v];
+}
+
+// ------------------------------------------------------------------ face term
+__device__ __forceinline__ void face_trace(const KernelArgs& a, int64_t cell, int combo, int g, double U[5]) {
+    const int NV = 5 * a.N;
+    for (int v = 0; v < 5; ++v) U[v] = 0;
+    const double* phi = a.Phi + ((int64_t)combo * a.Qf + g) * a.N;
+    for (int k = 0; k < a.N; ++k)
+        for (int v = 0; v < 5; ++v) U[v] += phi[k] * a.U[cidx(cell, 5 * k + v, NV)];
+}
+
+// ghost state of a boundary face point; false when the boundary type contributes nothing
+__device__ __forceinline__ bool ghost_state(const KernelArgs& a, int64_t f, int g, const double Uin[5],
+                                            double Ug[5]) {
+    const int bc = a.fbc[f];
+    if (bc == FVDG_BC_DIRICHLET) {
+        const double* ud = a.UD + ((int64_t)a.fdir[f] * a.Qf + g) * 5;
+        for (int v = 0; v < 5; ++v) Ug[v] = ud[v] + (ud[v] - Uin[v]);
+        return true;
+    }
+    if (bc == FVDG_BC_PSEUDO3DZ) {
+        for (int v = 0; v < 5; ++v) Ug[v] = Uin[v];
+        Ug[3] = -Uin[3];
+        return true;
+    }
+    return false;
+}
+
+// one thread per (face, face point): the numerical flux Fhat(U_L, U_R, n)
+__global__ void k_face_flux(KernelArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.F * a.Qf) return;
+    const int64_t f = t / a.Qf;
+    const int g = (int)(t - f * a.Qf);
+    const int64_t L = a.fcell[2 * f], Rc = a.fcell[2 * f + 1];
+    const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
+    double UL[5], UR[5], out[5] = {0, 0, 0, 0, 0};
+    face_trace(a, L, a.fcombo[2 * f], g, UL);
+    bool active = true;
+    if (Rc >= 0)
+        face_trace(a, Rc, a.fcombo[2 * f + 1], g, UR);
+    else
+        active = ghost_state(a, f, g, UL, UR);
+    if (active) num_flux(a.flux, UL, UR, n, a.gamma, out);
+    for (int v = 0; v < 5; ++v) a.Fh[t * 5 + v] = out[v];
+}
+
+// one thread per (element, test function j): add the four faces' contributions
+__global__ void k_face_test(KernelArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.E * a.N) return;
+    const int64_t e = t / a.N;
+    const int j = (int)(t - e * a.N);
+    const int NV = 5 * a.N;
+    const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
+    double r[5] = {0, 0, 0, 0, 0};
+    for (int l = 0; l < 4; ++l) {
+        const int32_t slot = a.cfaces[e * 4 + l];
+        const int64_t f = slot >> 1;
+        const int side = slot & 1;
+        const bool bnd = a.fcell[2 * f + 1] < 0;
+        if ((bnd && !do_bnd) || (!bnd && !do_int)) continue;
+        const int combo = a.fcombo[2 * f + side];
+        const double J = a.fgeo[4 * f + 3];
+        for (int g = 0; g < a.Qf; ++g) {
+            const double phi = a.Phi[((int64_t)combo * a.Qf + g) * a.N + j];
+            const double jw = a.wf[g] * J;
+            const double* fh = a.Fh + (f * a.Qf + g) * 5;
+            if (side == 0)
+                for (int v = 0; v < 5; ++v) r[v] += fh[v] * phi * jw;
+            else
+                for (int v = 0; v < 5; ++v) r[v] -= fh[v] * phi * jw;
+        }
+    }
+    const bool acc = (a.parts & FVDG_PART_CELLS) != 0;
+    for (int v = 0; v < 5; ++v) {
+        const int64_t i = cidx(e, 5 * j + v, NV);
+        a.R[i] = acc ? a.R[i] + r[v] : r[v];
+    }
+}
+
+// ------------------------------------------------------------------ SSP-RK stage update
+// out = a0*U0 + b0*(Us + dt * (-R / M)),  M = |det A_e| * mass_hat[k]  (padding lanes -> 0)
+__global__ void k_rk_update(const double* __restrict__ U0, const double* __restrict__ Us,
+                            const double* __restrict__ R, double* __restrict__ out, const double* __restrict__ cgeo,
+                            const double* __restrict__ mhat, int64_t E, int N, double a0, double b0, double dt) {
+    const int NV = 5 * N;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t Epad = (E + 63) / 64 * 64;
+    if (i >= Epad * NV) return;
+    const int64_t tile = i / (NV * 64);
+    const int64_t rem = i - tile * NV * 64;
+    const int kv = (int)(rem >> 6);
+    const int64_t e = tile * 64 + (rem & 63);
+    if (e >= E) {
+        out[i] = 0.0;
+        return;
+    }
+    const double M = cgeo[e * 10 + 9] * mhat[kv / 5];
+    const double L = -R[i] / M;
+    const double s = Us[i] + dt * L;
+    out[i] = (a0 == 0.0) ? b0 * s : a0 * U0[i] + b0 * s;
+}
+
+// ------------------------------------------------------------------ reductions
+__global__ void k_max_wave_speed(KernelArgs a, double* __restrict__ part) {
+    __shared__ double sm[256];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (t < a.F * a.Qf) {
+        const int64_t f = t / a.Qf;
+        const int g = (int)(t - f * a.Qf);
+        double U[5];
+        face_trace(a, a.fcell[2 * f], a.fcombo[2 * f], g, U);
+        m = euler_wave_speed(U, a.gamma);
+        if (a.fcell[2 * f + 1] >= 0) {
+            face_trace(a, a.fcell[2 * f + 1], a.fcombo[2 * f + 1], g, U);
+            m = fmax(m, euler_wave_speed(U, a.gamma));
+        }
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// integral of each conserved variable: |det A_e| * c_{e,0,v} * (1/6), phi_0 = 1
+__global__ void k_conservation(const double* __restrict__ U, const double* __restrict__ cgeo, int64_t E, int N,
+                               double* __restrict__ part) {
+    __shared__ double sm[5][256];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NV = 5 * N;
+    for (int v = 0; v < 5; ++v) sm[v][threadIdx.x] = (e < E) ? cgeo[e * 10 + 9] * U[cidx(e, v, NV)] / 6.0 : 0.0;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int v = 0; v < 5; ++v) sm[v][threadIdx.x] += sm[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < 5; ++v) part[blockIdx.x * 5 + v] = sm[v][0];
+}

@@ -1,0 +1,552 @@
+// Device half of libfvdg.so: the solver context (one GPU, one stream), device
+// copies of mesh geometry and reference tables, and the residual / SSP-RK3
+// kernels for gfx950.
+//
+// Device state layout ("tiled SoA"): coefficient kv = 5k+v of element e lives at
+//   U[(e/64) * (5N*64) + kv*64 + e%64]
+// i.e. 64-element tiles, one 512-byte row per coefficient: a wavefront that owns
+// 64 consecutive elements reads every coefficient row as one contiguous 512 B
+// burst.  The host ABI uses the reference's AoS layout and is transposed on the
+// device at upload/download.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "fvdg_internal.hpp"
+#include "fvdg_kernels.hpp"
+
+using namespace fvdg;
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return fail(FVDG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(&p, count * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& v, hipStream_t s) {
+        hipError_t e = alloc(v.size());
+        if (e != hipSuccess || v.empty()) return e;
+        e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipStreamSynchronize(s);
+    }
+};
+
+}  // namespace
+
+struct fvdg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // scheme
+    int order = -1, flux = FVDG_FLUX_LF, precision = FVDG_PREC_FP64;
+    double gamma = 1.4;
+    SchemeTables tab;
+    // mesh
+    int64_t E = 0, Epad = 0, F = 0, n_dir = 0;
+    bool have_mesh = false, have_state = false;
+    std::vector<int64_t> dir_faces;       // Dirichlet face ids, host copy
+    std::vector<double> dir_xyz;          // their face quadrature points
+    std::vector<double> cell_det_host;    // for the mass diagonal
+    // device geometry
+    DevBuf<double> cgeo;        // E*10: A^{-1} (9), |det A|
+    DevBuf<int32_t> cfaces;     // E*4: face*2 + side
+    DevBuf<int32_t> fcell;      // F*2
+    DevBuf<uint8_t> fcombo;     // F*2
+    DevBuf<double> fgeo;        // F*4: normal (3), 2*area
+    DevBuf<uint8_t> fbc;        // F: 255 interior, else BoundaryType
+    DevBuf<int32_t> fdir;       // F: Dirichlet slot or -1
+    DevBuf<double> UD;          // n_dir*Qf*5
+    // device tables
+    DevBuf<double> tV, tG, tWq, tPhi, tWf, tMhat;
+    // device state
+    DevBuf<double> U, Us, R, stage;  // stage: AoS staging for upload/download
+    DevBuf<double> Fq, Fh;           // pointwise flux scratch: E*Q*15, F*Qf*5
+    DevBuf<double> red;              // reduction partials
+    int64_t n_red = 0;
+
+    ~fvdg_ctx() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int check_ready(fvdg_ctx* c, bool need_state) {
+    if (!c) return fail(FVDG_ERR_ARG, "null context");
+    if (c->order < 0) return fail(FVDG_ERR_STATE, "fvdg_set_scheme has not been called");
+    if (!c->have_mesh) return fail(FVDG_ERR_STATE, "fvdg_set_mesh has not been called");
+    if (need_state && !c->have_state) return fail(FVDG_ERR_STATE, "no state uploaded");
+    return FVDG_OK;
+}
+
+inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int upload_tables(fvdg_ctx* c) {
+    const SchemeTables& T = c->tab;
+    HIP_TRY(c->tV.upload(T.V, c->stream));
+    HIP_TRY(c->tG.upload(T.G, c->stream));
+    HIP_TRY(c->tWq.upload(T.vq_w, c->stream));
+    HIP_TRY(c->tPhi.upload(T.Phi, c->stream));
+    HIP_TRY(c->tWf.upload(T.fq_w, c->stream));
+    HIP_TRY(c->tMhat.upload(T.mass_hat, c->stream));
+    return FVDG_OK;
+}
+
+int alloc_state(fvdg_ctx* c) {
+    if (c->order < 0 || !c->have_mesh) return FVDG_OK;
+    const int N = c->tab.N;
+    const size_t ndof = (size_t)c->Epad * 5 * N;
+    HIP_TRY(c->U.alloc(ndof));
+    HIP_TRY(c->Us.alloc(ndof));
+    HIP_TRY(c->R.alloc(ndof));
+    HIP_TRY(c->stage.alloc((size_t)c->E * 5 * N));
+    HIP_TRY(hipMemsetAsync(c->U.p, 0, ndof * sizeof(double), c->stream));
+    HIP_TRY(hipMemsetAsync(c->Us.p, 0, ndof * sizeof(double), c->stream));
+    HIP_TRY(hipMemsetAsync(c->R.p, 0, ndof * sizeof(double), c->stream));
+    HIP_TRY(c->Fq.alloc((size_t)c->E * c->tab.Q * 15));
+    HIP_TRY(c->Fh.alloc((size_t)c->F * c->tab.Qf * 5));
+    HIP_TRY(c->UD.alloc((size_t)std::max<int64_t>(c->n_dir, 1) * c->tab.Qf * 5));
+    HIP_TRY(hipMemsetAsync(c->UD.p, 0, c->UD.n * sizeof(double), c->stream));
+    c->n_red = std::max<int64_t>(nblocks(std::max(c->E, c->F * c->tab.Qf), 256), 1);
+    HIP_TRY(c->red.alloc((size_t)c->n_red * 5));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->have_state = false;
+    return FVDG_OK;
+}
+
+// ---------------------------------------------------------------- launches
+int launch_residual(fvdg_ctx* c, const double* U, double* R, int parts, double* ms_vol, double* ms_surf) {
+    const SchemeTables& T = c->tab;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    const bool timed = ms_vol || ms_surf;
+    if (timed)
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    if (timed) HIP_TRY(hipEventRecord(ev[0], c->stream));
+    KernelArgs a;
+    a.E = c->E;
+    a.F = c->F;
+    a.N = T.N;
+    a.Q = T.Q;
+    a.Qf = T.Qf;
+    a.gamma = c->gamma;
+    a.flux = c->flux;
+    a.parts = parts;
+    a.U = U;
+    a.R = R;
+    a.cgeo = c->cgeo.p;
+    a.cfaces = c->cfaces.p;
+    a.fcell = c->fcell.p;
+    a.fcombo = c->fcombo.p;
+    a.fgeo = c->fgeo.p;
+    a.fbc = c->fbc.p;
+    a.fdir = c->fdir.p;
+    a.UD = c->UD.p;
+    a.V = c->tV.p;
+    a.G = c->tG.p;
+    a.wq = c->tWq.p;
+    a.Phi = c->tPhi.p;
+    a.wf = c->tWf.p;
+    a.Fq = c->Fq.p;
+    a.Fh = c->Fh.p;
+    const int bs = 256;
+    if (parts & FVDG_PART_CELLS) {
+        hipLaunchKernelGGL(k_vol_flux, dim3(nblocks(c->E * T.Q, bs)), dim3(bs), 0, c->stream, a);
+        hipLaunchKernelGGL(k_vol_test, dim3(nblocks(c->E * T.N, bs)), dim3(bs), 0, c->stream, a);
+    }
+    if (timed) HIP_TRY(hipEventRecord(ev[1], c->stream));
+    if (parts & (FVDG_PART_INTERNALS | FVDG_PART_BOUNDARYS)) {
+        hipLaunchKernelGGL(k_face_flux, dim3(nblocks(c->F * T.Qf, bs)), dim3(bs), 0, c->stream, a);
+    }
+    hipLaunchKernelGGL(k_face_test, dim3(nblocks(c->E * T.N, bs)), dim3(bs), 0, c->stream, a);
+    HIP_TRY(hipGetLastError());
+    if (timed) {
+        HIP_TRY(hipEventRecord(ev[2], c->stream));
+        HIP_TRY(hipEventSynchronize(ev[2]));
+        float t01 = 0, t12 = 0;
+        HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]));
+        if (ms_vol) *ms_vol += t01;
+        if (ms_surf) *ms_surf += t12;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    return FVDG_OK;
+}
+
+int launch_update(fvdg_ctx* c, const double* U0, const double* Us, const double* R, double* out, double a0,
+                  double b0, double dt) {
+    const int64_t n = c->Epad * 5 * c->tab.N;
+    hipLaunchKernelGGL(k_rk_update, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, U0, Us, R, out, c->cgeo.p,
+                       c->tMhat.p, c->E, c->tab.N, a0, b0, dt);
+    HIP_TRY(hipGetLastError());
+    return FVDG_OK;
+}
+
+// one SSP-RK3 step on c->U (Shu & Osher 1988):
+//   U1 = U + dt L(U);  U2 = 3/4 U + 1/4 (U1 + dt L(U1));  U <- 1/3 U + 2/3 (U2 + dt L(U2))
+int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
+    static const double A[3] = {0.0, 0.75, 1.0 / 3.0};
+    static const double B[3] = {1.0, 0.25, 2.0 / 3.0};
+    for (int s = 0; s < 3; ++s) {
+        const double* src = (s == 0) ? c->U.p : c->Us.p;
+        double* dst = (s == 2) ? c->U.p : c->Us.p;
+        int rc = launch_residual(c, src, c->R.p, FVDG_PART_ALL, ms ? &ms[1] : nullptr, ms ? &ms[2] : nullptr);
+        if (rc) return rc;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (ms) {
+            HIP_TRY(hipEventCreate(&e0));
+            HIP_TRY(hipEventCreate(&e1));
+            HIP_TRY(hipEventRecord(e0, c->stream));
+        }
+        rc = launch_update(c, c->U.p, src, c->R.p, dst, A[s], B[s], dt);
+        if (rc) return rc;
+        if (ms) {
+            HIP_TRY(hipEventRecord(e1, c->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+            ms[3] += t;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+    return FVDG_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI (device part)
+extern "C" {
+
+int fvdg_create(fvdg_ctx** out, int device) {
+    if (!out) return fail(FVDG_ERR_ARG, "null output pointer");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(FVDG_ERR_ARG, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<fvdg_ctx> c(new fvdg_ctx);
+    c->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+    return FVDG_OK;
+}
+
+void fvdg_destroy(fvdg_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c;
+}
+
+int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precision) {
+    if (!c) return fail(FVDG_ERR_ARG, "null context");
+    if (order < 0 || order > kMaxOrder) return fail(FVDG_ERR_UNSUPPORTED, "order must be 0..4");
+    if (flux != FVDG_FLUX_LF && flux != FVDG_FLUX_ROE) return fail(FVDG_ERR_ARG, "unknown flux");
+    if (!(gamma > 1.0)) return fail(FVDG_ERR_ARG, "gamma must exceed 1");
+    if (precision != FVDG_PREC_FP64) return fail(FVDG_ERR_UNSUPPORTED, "only FVDG_PREC_FP64 is available");
+    HIP_TRY(hipSetDevice(c->device));
+    const bool reorder = c->order != order;
+    c->flux = flux;
+    c->gamma = gamma;
+    c->precision = precision;
+    if (reorder) {
+        c->order = order;
+        c->tab = build_tables(order);
+        int rc = upload_tables(c);
+        if (rc) return rc;
+        rc = alloc_state(c);
+        if (rc) return rc;
+    }
+    return FVDG_OK;
+}
+
+int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
+    if (!c || !m) return fail(FVDG_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const ComputingMesh& cm = m->cm;
+    const int64_t E = cm.m_cells.size(), F = cm.m_faces.size();
+    if (E == 0) return fail(FVDG_ERR_MESH, "empty mesh");
+    if (E > (int64_t(1) << 30) || F > (int64_t(1) << 30)) return fail(FVDG_ERR_UNSUPPORTED, "mesh too large for 32-bit indices");
+    std::vector<double> cgeo(E * 10);
+    std::vector<int32_t> cfaces(E * 4);
+    std::vector<double> det(E);
+    for (int64_t e = 0; e < E; ++e) {
+        const auto& t = cm.m_cells[e];
+        for (int i = 0; i < 9; ++i) cgeo[e * 10 + i] = t.m_invJac[i];
+        cgeo[e * 10 + 9] = t.compute_jacobian_det();
+        det[e] = cgeo[e * 10 + 9];
+    }
+    std::vector<int32_t> fcell(F * 2);
+    std::vector<uint8_t> fcombo(F * 2), fbc(F);
+    std::vector<double> fgeo(F * 4);
+    std::vector<int32_t> fdir(F, -1);
+    std::vector<int64_t> dir_faces;
+    for (int64_t f = 0; f < F; ++f) {
+        const auto& Fc = cm.m_faces[f];
+        const bool bnd = Fc.is_boundary();
+        fcell[2 * f] = (int32_t)Fc.m_neighbor_cells[0];
+        fcell[2 * f + 1] = bnd ? -1 : (int32_t)Fc.m_neighbor_cells[1];
+        for (int s = 0; s < 2; ++s) {
+            const int cb = (s == 1 && bnd) ? 0 : face_combo(Fc, s);
+            if (cb < 0) return fail(FVDG_ERR_MESH, "face natural coordinates are not a vertex permutation");
+            fcombo[2 * f + s] = (uint8_t)cb;
+        }
+        for (int d = 0; d < 3; ++d) fgeo[4 * f + d] = Fc.m_normal[d];
+        fgeo[4 * f + 3] = Fc.compute_jacobian_det();
+        const BoundaryType bt = f < (int64_t)cm.m_boundaryTypes.size() ? cm.m_boundaryTypes[f] : BoundaryType::Dirichlet;
+        fbc[f] = bnd ? (uint8_t)bt : (uint8_t)255;
+        if (bnd && bt == BoundaryType::Dirichlet) {
+            fdir[f] = (int32_t)dir_faces.size();
+            dir_faces.push_back(f);
+        }
+    }
+    for (int64_t e = 0; e < E; ++e) {
+        const auto& t = cm.m_cells[e];
+        for (int l = 0; l < 4; ++l) {
+            const uInt f = t.m_faces[l];
+            const auto& Fc = cm.m_faces[f];
+            int side = (Fc.m_neighbor_cells[0] == (uInt)e) ? 0 : 1;
+            // a cell that is its own periodic neighbour: pick the side whose nodes are this local face
+            if (!Fc.is_boundary() && Fc.m_neighbor_cells[0] == Fc.m_neighbor_cells[1]) {
+                const vector3u& n0 = Fc.m_nodes;
+                bool has_opp = (n0[0] == t.m_nodes[l] || n0[1] == t.m_nodes[l] || n0[2] == t.m_nodes[l]);
+                side = has_opp ? 1 : 0;
+            }
+            cfaces[e * 4 + l] = (int32_t)(2 * f + side);
+        }
+    }
+    c->E = E;
+    c->Epad = (E + 63) / 64 * 64;
+    c->F = F;
+    c->n_dir = dir_faces.size();
+    c->dir_faces = dir_faces;
+    c->dir_xyz.clear();
+    c->cell_det_host = det;
+    HIP_TRY(c->cgeo.upload(cgeo, c->stream));
+    HIP_TRY(c->cfaces.upload(cfaces, c->stream));
+    HIP_TRY(c->fcell.upload(fcell, c->stream));
+    HIP_TRY(c->fcombo.upload(fcombo, c->stream));
+    HIP_TRY(c->fgeo.upload(fgeo, c->stream));
+    HIP_TRY(c->fbc.upload(fbc, c->stream));
+    HIP_TRY(c->fdir.upload(fdir, c->stream));
+    // keep the Dirichlet faces' corner coordinates to place their quadrature points
+    c->dir_xyz.resize(c->n_dir * 9);
+    for (int64_t i = 0; i < c->n_dir; ++i)
+        for (int k = 0; k < 3; ++k)
+            for (int d = 0; d < 3; ++d) c->dir_xyz[i * 9 + 3 * k + d] = cm.m_faces[dir_faces[i]].m_coords[k][d];
+    c->have_mesh = true;
+    c->have_state = false;
+    return alloc_state(c);
+}
+
+int fvdg_state_upload(fvdg_ctx* c, const double* Uh) {
+    int rc = check_ready(c, false);
+    if (rc) return rc;
+    if (!Uh) return fail(FVDG_ERR_ARG, "null state");
+    HIP_TRY(hipSetDevice(c->device));
+    const int NV = 5 * c->tab.N;
+    HIP_TRY(hipMemcpyAsync(c->stage.p, Uh, (size_t)c->E * NV * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_aos_to_tiled, dim3(nblocks(c->E * NV, 256)), dim3(256), 0, c->stream, c->stage.p, c->U.p, c->E,
+                       NV);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return FVDG_OK;
+}
+
+int fvdg_state_download(fvdg_ctx* c, double* Uh) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (!Uh) return fail(FVDG_ERR_ARG, "null state");
+    HIP_TRY(hipSetDevice(c->device));
+    const int NV = 5 * c->tab.N;
+    hipLaunchKernelGGL(k_tiled_to_aos, dim3(nblocks(c->E * NV, 256)), dim3(256), 0, c->stream, c->U.p, c->stage.p, c->E,
+                       NV);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(Uh, c->stage.p, (size_t)c->E * NV * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
+
+int fvdg_dirichlet_points(fvdg_ctx* c, int64_t* n_faces, int64_t* face_ids, double* xyz) {
+    int rc = check_ready(c, false);
+    if (rc) return rc;
+    if (n_faces) *n_faces = c->n_dir;
+    const int Qf = c->tab.Qf;
+    for (int64_t i = 0; i < c->n_dir; ++i) {
+        if (face_ids) face_ids[i] = c->dir_faces[i];
+        if (xyz)
+            for (int g = 0; g < Qf; ++g) {
+                const double u = c->tab.fq_pts[2 * g], v = c->tab.fq_pts[2 * g + 1];
+                for (int d = 0; d < 3; ++d) {
+                    const double* p = &c->dir_xyz[i * 9];
+                    xyz[(i * Qf + g) * 3 + d] = p[d] * (1 - u - v) + p[3 + d] * u + p[6 + d] * v;
+                }
+            }
+    }
+    return FVDG_OK;
+}
+
+int fvdg_set_dirichlet_states(fvdg_ctx* c, const double* UDh) {
+    int rc = check_ready(c, false);
+    if (rc) return rc;
+    if (c->n_dir == 0) return FVDG_OK;
+    if (!UDh) return fail(FVDG_ERR_ARG, "null Dirichlet states");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpyAsync(c->UD.p, UDh, (size_t)c->n_dir * c->tab.Qf * 5 * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
+
+int fvdg_residual(fvdg_ctx* c, double t, int parts, double* Rh) {
+    (void)t;  // time enters only through the Dirichlet states, which the caller sets
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (parts & ~FVDG_PART_ALL) return fail(FVDG_ERR_ARG, "unknown residual part bits");
+    HIP_TRY(hipSetDevice(c->device));
+    rc = launch_residual(c, c->U.p, c->R.p, parts, nullptr, nullptr);
+    if (rc) return rc;
+    if (Rh) {
+        const int NV = 5 * c->tab.N;
+        hipLaunchKernelGGL(k_tiled_to_aos, dim3(nblocks(c->E * NV, 256)), dim3(256), 0, c->stream, c->R.p, c->stage.p,
+                           c->E, NV);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(Rh, c->stage.p, (size_t)c->E * NV * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
+
+int fvdg_mass_diagonal(fvdg_ctx* c, double* M) {
+    int rc = check_ready(c, false);
+    if (rc) return rc;
+    if (!M) return fail(FVDG_ERR_ARG, "null output");
+    const int N = c->tab.N;
+    // reference driver form b(5i+k) += phi_i^2 * w_q * |det| summed over q (VortexBatch.cpp:155-163)
+    for (int64_t e = 0; e < c->E; ++e)
+        for (int k = 0; k < N; ++k) {
+            double s = 0;
+            for (int q = 0; q < c->tab.Q; ++q) {
+                const double ph = c->tab.V[q * N + k];
+                s += ph * ph * c->tab.vq_w[q] * c->cell_det_host[e];
+            }
+            for (int v = 0; v < 5; ++v) M[e * 5 * N + 5 * k + v] = s;
+        }
+    return FVDG_OK;
+}
+
+int fvdg_ssprk3_step(fvdg_ctx* c, double dt, int n_steps) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (n_steps < 0 || !(dt >= 0)) return fail(FVDG_ERR_ARG, "bad dt or step count");
+    HIP_TRY(hipSetDevice(c->device));
+    for (int i = 0; i < n_steps; ++i) {
+        rc = ssprk3_once(c, dt, nullptr);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
+
+int fvdg_max_wave_speed(fvdg_ctx* c, double* out) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (!out) return fail(FVDG_ERR_ARG, "null output");
+    HIP_TRY(hipSetDevice(c->device));
+    KernelArgs a;
+    a.E = c->E;
+    a.F = c->F;
+    a.N = c->tab.N;
+    a.Qf = c->tab.Qf;
+    a.gamma = c->gamma;
+    a.U = c->U.p;
+    a.fcell = c->fcell.p;
+    a.fcombo = c->fcombo.p;
+    a.Phi = c->tPhi.p;
+    const int64_t n = c->F * c->tab.Qf;
+    const unsigned nb = nblocks(n, 256);
+    hipLaunchKernelGGL(k_max_wave_speed, dim3(nb), dim3(256), 0, c->stream, a, c->red.p);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(nb);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->red.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double m = 0;
+    for (double v : part) m = std::max(m, v);
+    *out = m;
+    return FVDG_OK;
+}
+
+int fvdg_conservation_sums(fvdg_ctx* c, double out[5]) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (!out) return fail(FVDG_ERR_ARG, "null output");
+    HIP_TRY(hipSetDevice(c->device));
+    const unsigned nb = nblocks(c->E, 256);
+    hipLaunchKernelGGL(k_conservation, dim3(nb), dim3(256), 0, c->stream, c->U.p, c->cgeo.p, c->E, c->tab.N, c->red.p);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(nb * 5);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->red.p, nb * 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int v = 0; v < 5; ++v) {
+        double s = 0;
+        for (unsigned b = 0; b < nb; ++b) s += part[b * 5 + v];
+        out[v] = s;
+    }
+    return FVDG_OK;
+}
+
+int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (!ms || n_stages <= 0 || n_stages % 3) return fail(FVDG_ERR_ARG, "n_stages must be a positive multiple of 3");
+    HIP_TRY(hipSetDevice(c->device));
+    for (int i = 0; i < 4; ++i) ms[i] = 0;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    // pass 1: whole stages, no per-kernel events inside (the throughput figure)
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int s = 0; s < n_stages / 3; ++s) {
+        rc = ssprk3_once(c, dt, nullptr);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    ms[0] = t / n_stages;
+    // pass 2: per-kernel split (events between kernels)
+    double split[4] = {0, 0, 0, 0};
+    for (int s = 0; s < n_stages / 3; ++s) {
+        rc = ssprk3_once(c, dt, split);
+        if (rc) return rc;
+    }
+    for (int i = 1; i < 4; ++i) ms[i] = split[i] / n_stages;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return FVDG_OK;
+}
+
+}  // extern "C"

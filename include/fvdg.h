@@ -1,0 +1,149 @@
+/*
+ * fvdg.h -- C ABI of the MI355X explicit DG Euler solver (libfvdg.so).
+ *
+ * This is the only boundary between host code and the GPU.  Plain pointers,
+ * sizes and status codes; no HIP or C++ types cross it.  Every call returns
+ * FVDG_OK (0) or a negative status; fvdg_last_error() gives the message of
+ * the last failure on the calling thread.
+ *
+ * Reference interface replaced (the reference is a C++ template library with no
+ * FFI; these calls are what its ExplicitConvection path needs):
+ *   fvdg_mesh_*              ComputingMesh(GeneralMesh) + OrthHexMesh + split_hex*_scan
+ *                            (include/Mesh/ComputingMesh.h:98-109, src/Mesh/ComputingMesh.cpp:205-263,
+ *                             include/Mesh/OrthHexMesh.h:7-106, include/Mesh/GeneralMesh.h:24-25)
+ *   fvdg_residual            ExplicitConvection<Order,Flux>::eval / eval_cells / eval_internals /
+ *                            eval_boundarys (include/DG/DG_Schemes/ExplicitConvection.h:32-43,
+ *                             src/DG/DG_Schemes/ExplicitConvection.cpp:21-272)
+ *   fvdg_mass_diagonal       the driver's diagonal mass b[cell](5i+k)
+ *                            (examples/Vortex/123/VortexBatch.cpp:153-166)
+ *   fvdg_ssprk3_step         the driver's explicit loop x += dt * (-1/b * eval(...))
+ *                            (examples/Vortex/123/VortexBatch.cpp:183-208), as Shu-Osher SSP-RK3
+ *   fvdg_project_analytic    DGBasisEvaluator<P>::func2coef of the vortex exact solution
+ *                            (include/DG/DG_Basis/DG_Basis_Evaluator.h:33-59, examples/Vortex/problem.h:17-110)
+ *
+ * State layout at this boundary is the reference's LongVector<5N>: element e,
+ * mode k, variable v (rho, rho u, rho v, rho w, rho E) at U[e*5N + 5k + v].
+ */
+#ifndef FVDG_H
+#define FVDG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fvdg_mesh fvdg_mesh; /* host mesh: topology + geometry */
+typedef struct fvdg_ctx fvdg_ctx;   /* one GPU, one stream, one mesh, one scheme */
+
+/* status codes */
+#define FVDG_OK 0
+#define FVDG_ERR_ARG (-1)
+#define FVDG_ERR_HIP (-2)
+#define FVDG_ERR_MESH (-3)
+#define FVDG_ERR_STATE (-4)
+#define FVDG_ERR_UNSUPPORTED (-5)
+#define FVDG_ERR_NONFINITE (-6)
+
+/* numerical flux */
+#define FVDG_FLUX_LF 0  /* Lax-Friedrichs with lambda = max(|u|+c) (ExplicitConvection.cpp:232-240) */
+#define FVDG_FLUX_ROE 1 /* Roe with Harten entropy fix (no counterpart in the reference) */
+
+/* arithmetic of the volume/face contractions (state and pointwise flux stay FP64) */
+#define FVDG_PREC_FP64 0
+#define FVDG_PREC_FP32 1
+#define FVDG_PREC_BF16X3 2
+
+/* boundary types: the numbering of BoundaryType (include/Mesh/ComputingMesh.h) */
+#define FVDG_BC_DIRICHLET 0
+#define FVDG_BC_PSEUDO3DZ 5
+
+/* residual parts (bit mask): eval_cells, eval_internals, eval_boundarys */
+#define FVDG_PART_CELLS 1
+#define FVDG_PART_INTERNALS 2
+#define FVDG_PART_BOUNDARYS 4
+#define FVDG_PART_ALL 7
+
+/* built-in analytic fields for fvdg_project_analytic / fvdg_analytic_error */
+#define FVDG_FIELD_VORTEX 0 /* params[8]: gamma, strength, x0, y0, u_inf, v_inf, Lx, Ly (periods, 0 = none) */
+#define FVDG_FIELD_TGV 1    /* params[4]: gamma, rho0, V0, Mach */
+#define FVDG_FIELD_CONST 2  /* params[5]: rho, rho u, rho v, rho w, rho E */
+
+const char* fvdg_version(void);
+const char* fvdg_last_error(void);
+int fvdg_num_basis(int order);
+/* values val[N] and reference gradients grad[3N] (either may be NULL) of the order-`order`
+ * modal basis at reference point (x,y,z) (DGBasisEvaluator<P>::eval_all / grad_all,
+ * include/DG/DG_Basis/DG_Basis_Evaluator.h:14-31); returns N or a negative status */
+int fvdg_basis_eval(int order, double x, double y, double z, double* val, double* grad);
+/* points of the volume / face rule used at this order (reference AutoQuadSelector) */
+int fvdg_quadrature(int order, int* n_vol, double* vol_pts, double* vol_w, int* n_face, double* face_pts,
+                    double* face_w);
+
+/* ------------------------------ host mesh ------------------------------ */
+/* Box [lb,ub] of n[0]*n[1]*n[2] hexahedra split into 6 (split=6) or 5 (split=5) tets.
+ * periodic_mask bit d pairs the faces on x_d = lb_d and x_d = ub_d.  jitter moves every
+ * vertex class by U(-jitter*h, jitter*h) per coordinate (seeded, periodic images alike). */
+int fvdg_mesh_create_box(fvdg_mesh** out, const double lb[3], const double ub[3], const int64_t n[3], int split,
+                         int periodic_mask, double jitter, uint64_t seed);
+/* From arrays.  face_cells[2f+1] = -1 marks a boundary face.  face_nodes_r (nullable) gives
+ * the face nodes as numbered in face_cells[2f+1] (differs only across periodic boundaries). */
+int fvdg_mesh_create(fvdg_mesh** out, int64_t n_points, const double* points, int64_t n_cells,
+                     const int64_t* cell_nodes, int64_t n_faces, const int64_t* face_nodes,
+                     const int64_t* face_cells, const int64_t* face_nodes_r);
+int fvdg_mesh_sizes(const fvdg_mesh* m, int64_t* n_points, int64_t* n_cells, int64_t* n_faces);
+/* any output pointer may be NULL */
+int fvdg_mesh_export(const fvdg_mesh* m, double* points, int64_t* cell_nodes, int64_t* face_nodes,
+                     int64_t* face_cells, int64_t* face_nodes_r);
+/* per cell: inverse Jacobian (A^{-1}, row-major 9), |det A|, h = V/sum(face areas);
+ * per face: unit normal (cell 0 -> cell 1), area */
+int fvdg_mesh_geometry(const fvdg_mesh* m, double* cell_inv_jac, double* cell_det, double* cell_h,
+                       double* face_normal, double* face_area);
+int fvdg_mesh_set_boundary_types(fvdg_mesh* m, const uint8_t* bc); /* n_faces entries */
+int fvdg_mesh_get_boundary_types(const fvdg_mesh* m, uint8_t* bc);
+/* the reference example's tagging (examples/Vortex/tools.h:22-31):
+ * boundary face with |n_z| > 0.5 -> Pseudo3DZ, other boundary faces -> Dirichlet */
+int fvdg_mesh_tag_boundaries_pseudo3dz(fvdg_mesh* m);
+void fvdg_mesh_destroy(fvdg_mesh* m);
+
+/* Diagonal L2 projection of a built-in field at time t onto the order-`order` basis,
+ * written as U[e*5N + 5k + v] (host computation). */
+int fvdg_project_analytic(const fvdg_mesh* m, int order, int field, const double* params, int n_params, double t,
+                          double* U);
+/* Relative L2 error per variable of U against the field at time t, on the volume rule
+ * (sqrt(sum (U_h-U_s)^2 w|J| / sum U_s^2 w|J|), examples/Eular1D/Sod1D/Riemann1D.cpp:250-267). */
+int fvdg_analytic_error(const fvdg_mesh* m, int order, int field, const double* params, int n_params, double t,
+                        const double* U, double err[5]);
+
+/* ---------------------------- device solver ---------------------------- */
+int fvdg_create(fvdg_ctx** out, int device);
+void fvdg_destroy(fvdg_ctx* c);
+/* order 0..4; flux FVDG_FLUX_*; gamma > 1; precision FVDG_PREC_* */
+int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precision);
+/* Copies the mesh geometry to the device; the mesh object may be destroyed afterwards. */
+int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m);
+int fvdg_state_upload(fvdg_ctx* c, const double* U);
+int fvdg_state_download(fvdg_ctx* c, double* U);
+/* Dirichlet data: number of Dirichlet boundary faces, and for each its face id and the
+ * physical positions of its face quadrature points (xyz[(i*Qf+g)*3+d]); either pointer may be NULL. */
+int fvdg_dirichlet_points(fvdg_ctx* c, int64_t* n_faces, int64_t* face_ids, double* xyz);
+/* States U_D at those points, UD[(i*Qf+g)*5+v]; used by every later residual. */
+int fvdg_set_dirichlet_states(fvdg_ctx* c, const double* UD);
+/* R = sum of the selected parts of the residual of the current state, written as
+ * R[e*5N+5k+v] when R is not NULL (the device copy is kept either way). */
+int fvdg_residual(fvdg_ctx* c, double t, int parts, double* R);
+/* Diagonal mass M[e*5N+5k+v] = |det A_e| * sum_q w_q phi_k(xi_q)^2. */
+int fvdg_mass_diagonal(fvdg_ctx* c, double* M);
+/* n_steps SSP-RK3 steps of dU/dt = -M^{-1} R(U) on the device state. */
+int fvdg_ssprk3_step(fvdg_ctx* c, double dt, int n_steps);
+/* max over all face quadrature points of max(|u|+c) of the two traces */
+int fvdg_max_wave_speed(fvdg_ctx* c, double* out);
+/* sum over elements of |det A_e| * U_{e,k=0,v} / 6 = integral of each conserved variable */
+int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);
+/* Time n_stages SSP-RK3 stages (launches only, no host sync inside) with HIP events on the
+ * solver stream; ms[0] = total, ms[1..3] = volume, surface, update kernels (averaged per stage). */
+int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVDG_H */

@@ -1,0 +1,580 @@
+/*
+ * dg_oracle.c -- CPU ORACLE for the explicit DG Euler residual (TEST INFRASTRUCTURE).
+ *
+ * This file is a plain-C restatement of the reference's ExplicitConvection path
+ * and is used ONLY as a checker: by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg.  Nothing in the product (fve_dg_amd/, include/)
+ * links, loads or calls it.
+ *
+ * What it restates (reference file:line):
+ *   or_basis           DGBasis<k>::eval/grad, include/DG/DG_Basis/DG_Basis_Func.h:7-670, via the
+ *                      explicit Jacobi sum s^n P_n^{(a,0)}((2z-s)/s) = sum_m C(n+a,n-m) C(n,m) (z-s)^m z^(n-m)
+ *                      (a different formula from the library's recurrence)
+ *   or_flux, or_wave_speed, or_lf
+ *                      src/DG/DG_Flux/EulerPhysicalFlux.cpp:7-18, 46-55, 70-89;
+ *                      LF form of src/DG/DG_Schemes/ExplicitConvection.cpp:232-240
+ *   or_roe             Roe (1981) with Harten's entropy fix -- no counterpart in the reference
+ *   or_inverse3        include/Matrix/DenseMatrix.h:521-538
+ *   or_geometry        src/Mesh/ComputingMesh.cpp:40-76 (faces), :113-125 and :145-164 (tets)
+ *   or_residual        src/DG/DG_Schemes/ExplicitConvection.cpp:21-272 (cells, internals,
+ *                      boundarys), same loop nest: per point basis evaluation, a fresh 3x3
+ *                      inverse per (q, j), and with threads>1 OpenMP dynamic scheduling with
+ *                      atomic scatter as in the reference
+ *   or_mass            examples/Vortex/123/VortexBatch.cpp:153-166
+ *   or_ssprk3          Shu & Osher (1988) SSP-RK3 on dU/dt = -M^{-1} R (driver form
+ *                      examples/Vortex/123/VortexBatch.cpp:189-197 generalised to 3 stages)
+ *   or_vortex_state, or_project
+ *                      examples/Vortex/problem.h:17-110, include/DG/DG_Basis/DG_Basis_Evaluator.h:33-59
+ *
+ * Layouts: state U[e*5N + 5k + v] (reference LongVector<5N>); cells[4e+a]; faces[3f+k];
+ * face_cells[2f+s] with -1 for a boundary side; face_nodes_r[3f+k] = nodes seen from side 1.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/GaussQuadrature/quad_rules.inc"
+
+/* ------------------------------------------------------------------ quadrature */
+static const double TET15_P[][3] = FVDG_TET_DEGREE5POINTS15_PTS, TET15_W[] = FVDG_TET_DEGREE5POINTS15_W;
+static const double TET31_P[][3] = FVDG_TET_DEGREE7POINTS31_PTS, TET31_W[] = FVDG_TET_DEGREE7POINTS31_W;
+static const double TET57_P[][3] = FVDG_TET_DEGREE9POINTS57_PTS, TET57_W[] = FVDG_TET_DEGREE9POINTS57_W;
+static const double TET94_P[][3] = FVDG_TET_DEGREE11POINTS94_PTS, TET94_W[] = FVDG_TET_DEGREE11POINTS94_W;
+static const double TRI7_P[][2] = FVDG_TRI_DEGREE4POINTS7_PTS, TRI7_W[] = FVDG_TRI_DEGREE4POINTS7_W;
+static const double TRI24_P[][2] = FVDG_TRI_DEGREE10POINTS24_PTS, TRI24_W[] = FVDG_TRI_DEGREE10POINTS24_W;
+static const double TRI36_P[][2] = FVDG_TRI_DEGREE13POINTS36_PTS, TRI36_W[] = FVDG_TRI_DEGREE13POINTS36_W;
+
+/* AutoQuadSelector: include/GaussQuadrature/GaussLegendre.h:84-111, AutoQuadSelectorTri.h:1-22 */
+int or_tet_rule(int order, const double** pts, const double** w) {
+    switch (order) {
+        case 0: case 1: case 2: *pts = &TET15_P[0][0]; *w = TET15_W; return 15;
+        case 3: *pts = &TET31_P[0][0]; *w = TET31_W; return 31;
+        case 4: *pts = &TET57_P[0][0]; *w = TET57_W; return 57;
+        case 5: *pts = &TET94_P[0][0]; *w = TET94_W; return 94;
+        default: return 0;
+    }
+}
+int or_tri_rule(int order, const double** pts, const double** w) {
+    switch (order) {
+        case 0: case 1: *pts = &TRI7_P[0][0]; *w = TRI7_W; return 7;
+        case 2: case 3: case 4: *pts = &TRI24_P[0][0]; *w = TRI24_W; return 24;
+        case 5: *pts = &TRI36_P[0][0]; *w = TRI36_W; return 36;
+        default: return 0;
+    }
+}
+/* copy-out form for tests */
+int or_quadrature(int order, int face, double* pts, double* w) {
+    const double *p, *ww;
+    int n = face ? or_tri_rule(order, &p, &ww) : or_tet_rule(order, &p, &ww);
+    if (pts) memcpy(pts, p, sizeof(double) * n * (face ? 2 : 3));
+    if (w) memcpy(w, ww, sizeof(double) * n);
+    return n;
+}
+
+int or_num_basis(int order) { return (order + 1) * (order + 2) * (order + 3) / 6; }
+
+/* ------------------------------------------------------------------ basis */
+typedef struct { double v, x, y, z; } dual;
+static dual d_const(double c) { dual r = {c, 0, 0, 0}; return r; }
+static dual d_add(dual a, dual b) { dual r = {a.v + b.v, a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
+static dual d_sub(dual a, dual b) { dual r = {a.v - b.v, a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static dual d_mul(dual a, dual b) {
+    dual r = {a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y, a.z * b.v + a.v * b.z};
+    return r;
+}
+static dual d_scale(double c, dual a) { dual r = {c * a.v, c * a.x, c * a.y, c * a.z}; return r; }
+static dual d_pow(dual a, int m) {
+    dual r = d_const(1.0);
+    for (int i = 0; i < m; ++i) r = d_mul(r, a);
+    return r;
+}
+static double binom(int n, int k) {
+    if (k < 0 || k > n) return 0.0;
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+/* s^n P_n^{(alpha,0)}((2z - s)/s) = sum_m C(n+alpha, n-m) C(n, m) (z - s)^m z^(n-m) */
+static dual jacobi_sum(int n, int alpha, dual z, dual s) {
+    dual acc = d_const(0.0), zs = d_sub(z, s);
+    for (int m = 0; m <= n; ++m)
+        acc = d_add(acc, d_scale(binom(n + alpha, n - m) * binom(n, m), d_mul(d_pow(zs, m), d_pow(z, n - m))));
+    return acc;
+}
+
+/* values val[N] and gradients grad[3N] (may be NULL) of the order-`order` basis */
+void or_basis(int order, double x, double y, double z, double* val, double* grad) {
+    dual X = {x, 1, 0, 0}, Y = {y, 0, 1, 0}, Z = {z, 0, 0, 1}, one = d_const(1.0);
+    dual s = d_sub(d_sub(one, X), Y), r = d_sub(one, X);
+    int idx = 0;
+    for (int n = 0; n <= order; ++n)
+        for (int k = 0; k <= n; ++k)
+            for (int j = 0; j <= n - k; ++j) {
+                int i = n - j - k;
+                dual phi = d_mul(d_mul(jacobi_sum(k, 0, Z, s), jacobi_sum(j, 2 * k + 1, Y, r)),
+                                 jacobi_sum(i, 2 * j + 2 * k + 2, X, one));
+                if (val) val[idx] = phi.v;
+                if (grad) {
+                    grad[3 * idx] = phi.x;
+                    grad[3 * idx + 1] = phi.y;
+                    grad[3 * idx + 2] = phi.z;
+                }
+                ++idx;
+            }
+}
+
+/* ------------------------------------------------------------------ Euler algebra */
+static double positive(double v) { return v > 1e-16 ? v : 1e-16; }
+double or_pressure(const double U[5], double gamma) {
+    double rho = positive(U[0]);
+    double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, e = U[4] / rho;
+    return (gamma - 1) * (rho * e - 0.5 * rho * (u * u + v * v + w * w));
+}
+/* F[3v+d], row v = conserved variable, column d = direction */
+void or_flux(const double U[5], double gamma, double F[15]) {
+    double rho = positive(U[0]);
+    double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, e = U[4] / rho;
+    double p = or_pressure(U, gamma);
+    F[0] = rho * u;              F[1] = rho * v;              F[2] = rho * w;
+    F[3] = rho * u * u + p;      F[4] = rho * v * u;          F[5] = rho * w * u;
+    F[6] = rho * u * v;          F[7] = rho * v * v + p;      F[8] = rho * w * v;
+    F[9] = rho * u * w;          F[10] = rho * v * w;         F[11] = rho * w * w + p;
+    F[12] = u * (rho * e + p);   F[13] = v * (rho * e + p);   F[14] = w * (rho * e + p);
+}
+static double wave_mag(const double U[5], double gamma) {
+    double rho = positive(U[0]);
+    double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho;
+    double c = sqrt(gamma * positive(or_pressure(U, gamma) / positive(U[0])));
+    return sqrt(u * u + v * v + w * w) + c;
+}
+double or_wave_speed(const double UL[5], const double UR[5], double gamma) {
+    double a = wave_mag(UL, gamma), b = wave_mag(UR, gamma);
+    return a > b ? a : b;
+}
+static void flux_n(const double U[5], const double n[3], double gamma, double out[5]) {
+    double F[15];
+    or_flux(U, gamma, F);
+    for (int v = 0; v < 5; ++v) out[v] = F[3 * v] * n[0] + F[3 * v + 1] * n[1] + F[3 * v + 2] * n[2];
+}
+void or_lf(const double UL[5], const double UR[5], const double n[3], double gamma, double out[5]) {
+    double fl[5], fr[5];
+    flux_n(UL, n, gamma, fl);
+    flux_n(UR, n, gamma, fr);
+    double lam = or_wave_speed(UL, UR, gamma);
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fl[v] + fr[v] + lam * (UL[v] - UR[v]));
+}
+/* Roe flux 0.5(F_L n + F_R n) - 0.5 sum_k |l_k| alpha_k r_k with Harten's fix (delta = 0.1 a)
+ * on the acoustic waves.  signed_eigs != 0 returns instead sum_k l_k alpha_k r_k (no abs, no
+ * fix), which equals (F_R - F_L) n exactly for the Roe average (tests pin the decomposition). */
+void or_roe(const double UL[5], const double UR[5], const double n[3], double gamma, double out[5],
+            int signed_eigs) {
+    double rL = positive(UL[0]), rR = positive(UR[0]);
+    double pL = or_pressure(UL, gamma), pR = or_pressure(UR, gamma);
+    double uL[3] = {UL[1] / rL, UL[2] / rL, UL[3] / rL}, uR[3] = {UR[1] / rR, UR[2] / rR, UR[3] / rR};
+    double HL = (UL[4] + pL) / rL, HR = (UR[4] + pR) / rR;
+    double sL = sqrt(rL), sR = sqrt(rR);
+    double ut[3], q2 = 0, qn = 0;
+    for (int d = 0; d < 3; ++d) {
+        ut[d] = (sL * uL[d] + sR * uR[d]) / (sL + sR);
+        q2 += ut[d] * ut[d];
+        qn += ut[d] * n[d];
+    }
+    double Ht = (sL * HL + sR * HR) / (sL + sR);
+    double a2 = (gamma - 1) * (Ht - 0.5 * q2);
+    if (a2 < 1e-16) a2 = 1e-16;
+    double a = sqrt(a2), rt = sL * sR;
+    double dr = rR - rL, dp = pR - pL, du[3], dqn = 0;
+    for (int d = 0; d < 3; ++d) {
+        du[d] = uR[d] - uL[d];
+        dqn += du[d] * n[d];
+    }
+    double dut[3], ut_dut = 0;
+    for (int d = 0; d < 3; ++d) {
+        dut[d] = du[d] - dqn * n[d];
+        ut_dut += ut[d] * dut[d];
+    }
+    double al1 = (dp - rt * a * dqn) / (2 * a2), al5 = (dp + rt * a * dqn) / (2 * a2), al2 = dr - dp / a2;
+    double l1 = qn - a, l2 = qn, l5 = qn + a;
+    if (!signed_eigs) {
+        double delta = 0.1 * a;
+        l1 = fabs(l1) < delta ? (l1 * l1 + delta * delta) / (2 * delta) : fabs(l1);
+        l5 = fabs(l5) < delta ? (l5 * l5 + delta * delta) / (2 * delta) : fabs(l5);
+        l2 = fabs(l2);
+    }
+    double dd[5];
+    dd[0] = l1 * al1 + l2 * al2 + l5 * al5;
+    for (int k = 0; k < 3; ++k)
+        dd[1 + k] = l1 * al1 * (ut[k] - a * n[k]) + l2 * al2 * ut[k] + l2 * rt * dut[k] + l5 * al5 * (ut[k] + a * n[k]);
+    dd[4] = l1 * al1 * (Ht - a * qn) + l2 * al2 * 0.5 * q2 + l2 * rt * ut_dut + l5 * al5 * (Ht + a * qn);
+    if (signed_eigs) {
+        for (int v = 0; v < 5; ++v) out[v] = dd[v];
+        return;
+    }
+    double fl[5], fr[5];
+    flux_n(UL, n, gamma, fl);
+    flux_n(UR, n, gamma, fr);
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * dd[v];
+}
+static void num_flux(int kind, const double UL[5], const double UR[5], const double n[3], double gamma, double out[5]) {
+    if (kind == 1)
+        or_roe(UL, UR, n, gamma, out, 0);
+    else
+        or_lf(UL, UR, n, gamma, out);
+}
+
+/* ------------------------------------------------------------------ geometry */
+void or_inverse3(const double A[9], double R[9]) {
+#define a(i, j) A[3 * (i) + (j)]
+#define r(i, j) R[3 * (i) + (j)]
+    r(0, 0) = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1);
+    r(1, 0) = a(1, 2) * a(2, 0) - a(1, 0) * a(2, 2);
+    r(2, 0) = a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0);
+    r(0, 1) = a(2, 1) * a(0, 2) - a(2, 2) * a(0, 1);
+    r(1, 1) = a(2, 2) * a(0, 0) - a(2, 0) * a(0, 2);
+    r(2, 1) = a(2, 0) * a(0, 1) - a(2, 1) * a(0, 0);
+    r(0, 2) = a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1);
+    r(1, 2) = a(0, 2) * a(1, 0) - a(0, 0) * a(1, 2);
+    r(2, 2) = a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0);
+    double det = 0;
+    det += a(0, 0) * r(0, 0);
+    det += a(1, 0) * r(0, 1);
+    det += a(2, 0) * r(0, 2);
+    for (int i = 0; i < 9; ++i) R[i] /= det;
+#undef a
+#undef r
+}
+static void sub3(const double* a, const double* b, double* o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; }
+static void cross3(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* tet: A (rows p1-p0, p2-p0, p3-p0), |det| = 6 * (|(p1-p0).((p2-p0)x(p3-p0))| / 6).
+ * face: unit normal from cell 0 towards cell 1, area, and for each side the reference
+ * vertex (0..3) of each face node (natural coordinates). */
+void or_geometry(int64_t np, const double* pts, int64_t nc, const int64_t* cells, int64_t nf,
+                 const int64_t* fnodes, const int64_t* fcells, const int64_t* fnodes_r, double* cA, double* cdet,
+                 double* fnormal, double* farea, int* fvert) {
+    (void)np;
+    for (int64_t e = 0; e < nc; ++e) {
+        const double *p0 = pts + 3 * cells[4 * e], *p1 = pts + 3 * cells[4 * e + 1], *p2 = pts + 3 * cells[4 * e + 2],
+                     *p3 = pts + 3 * cells[4 * e + 3];
+        double v1[3], v2[3], v3[3], c[3];
+        sub3(p1, p0, v1);
+        sub3(p2, p0, v2);
+        sub3(p3, p0, v3);
+        cross3(v2, v3, c);
+        double vol = fabs(dot3(v1, c)) / 6.0;
+        if (cdet) cdet[e] = vol * 6.0;
+        if (cA)
+            for (int d = 0; d < 3; ++d) {
+                cA[9 * e + d] = v1[d];
+                cA[9 * e + 3 + d] = v2[d];
+                cA[9 * e + 6 + d] = v3[d];
+            }
+    }
+    for (int64_t f = 0; f < nf; ++f) {
+        const int64_t* fn = fnodes + 3 * f;
+        const int64_t* fr = fnodes_r ? fnodes_r + 3 * f : fn;
+        const double *p0 = pts + 3 * fn[0], *p1 = pts + 3 * fn[1], *p2 = pts + 3 * fn[2];
+        double v1[3], v2[3], n[3];
+        sub3(p1, p0, v1);
+        sub3(p2, p0, v2);
+        cross3(v1, v2, n);
+        int64_t c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
+        int periodic = c1 >= 0 && (fr[0] != fn[0] || fr[1] != fn[1] || fr[2] != fn[2]);
+        if (c1 < 0 || periodic) {
+            double p3[3] = {0, 0, 0}, v3[3], cc[3];
+            for (int a = 0; a < 4; ++a) {
+                int64_t v = cells[4 * c0 + a];
+                if (v != fn[0] && v != fn[1] && v != fn[2]) memcpy(p3, pts + 3 * v, sizeof p3);
+            }
+            sub3(p3, p0, v3);
+            cross3(v1, v2, cc);
+            if (dot3(cc, v3) * dot3(cc, n) > 0) n[0] = -n[0], n[1] = -n[1], n[2] = -n[2];
+        } else {
+            double g0[3] = {0, 0, 0}, g1[3] = {0, 0, 0}, dg[3];
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    g0[d] += pts[3 * cells[4 * c0 + a] + d];
+                    g1[d] += pts[3 * cells[4 * c1 + a] + d];
+                }
+            sub3(g1, g0, dg);
+            if (dot3(n, dg) < 0) n[0] = -n[0], n[1] = -n[1], n[2] = -n[2];
+        }
+        double len = sqrt(dot3(n, n));
+        if (farea) farea[f] = len * 0.5;
+        if (fnormal)
+            for (int d = 0; d < 3; ++d) fnormal[3 * f + d] = n[d] / len;
+        if (fvert)
+            for (int s = 0; s < 2; ++s)
+                for (int k = 0; k < 3; ++k) {
+                    int64_t c = fcells[2 * f + s], node = s ? fr[k] : fn[k];
+                    int vtx = -1;
+                    if (c >= 0)
+                        for (int a = 0; a < 4; ++a)
+                            if (cells[4 * c + a] == node) vtx = a;
+                    fvert[6 * f + 3 * s + k] = vtx;
+                }
+    }
+}
+
+/* face point (u,v) of a face whose nodes are reference vertices vt[0..2] -> reference coords,
+ * nc0*(1-u-v) + nc1*u + nc2*v (ExplicitConvection.cpp:129-134) */
+static void face_to_cell(const int* vt, double u, double v, double xi[3]) {
+    double nc[3][3] = {{0}};
+    for (int k = 0; k < 3; ++k)
+        if (vt[k] > 0) nc[k][vt[k] - 1] = 1.0;
+    for (int d = 0; d < 3; ++d) xi[d] = nc[0][d] * (1 - u - v) + nc[1][d] * u + nc[2][d] * v;
+}
+
+/* ------------------------------------------------------------------ residual */
+typedef struct {
+    int order, N, Q, Qf, flux, parts, threads;
+    double gamma;
+    const double *qp, *qw, *fp, *fw;
+} scheme_t;
+
+static void add_scaled(double* dst, const double* src, double s, int n, int atomic_mode) {
+    for (int i = 0; i < n; ++i) {
+        if (atomic_mode) {
+#pragma omp atomic update
+            dst[i] += src[i] * s;
+        } else {
+            dst[i] += src[i] * s;
+        }
+    }
+}
+
+/*
+ * R = cells + internals + boundarys for the selected parts (1, 2, 4), reference form.
+ * bc[f] for boundary faces: 0 Dirichlet (ghost 2 U_D - U_in, U_D from UD[(i*Qf+g)*5+v]
+ * for the i-th Dirichlet face in face order), 5 Pseudo3DZ (mirror rho w), others nothing.
+ * flux: 0 LF, 1 Roe.  threads > 1: OpenMP dynamic schedule + atomics (reference mode).
+ */
+int or_residual(int order, int flux, double gamma, int64_t np, const double* pts, int64_t nc, const int64_t* cells,
+                int64_t nf, const int64_t* fnodes, const int64_t* fcells, const int64_t* fnodes_r, const uint8_t* bc,
+                const double* UD, const double* U, int parts, double* R, int threads) {
+    scheme_t S;
+    S.order = order;
+    S.N = or_num_basis(order);
+    S.Q = or_tet_rule(order, &S.qp, &S.qw);
+    S.Qf = or_tri_rule(order, &S.fp, &S.fw);
+    if (!S.Q || !S.Qf) return -1;
+    const int N = S.N, NV = 5 * N;
+    double* cA = (double*)malloc(sizeof(double) * 9 * nc);
+    double* cdet = (double*)malloc(sizeof(double) * nc);
+    double* fn = (double*)malloc(sizeof(double) * 3 * nf);
+    double* fa = (double*)malloc(sizeof(double) * nf);
+    int* fv = (int*)malloc(sizeof(int) * 6 * nf);
+    int64_t* dslot = (int64_t*)malloc(sizeof(int64_t) * nf);
+    or_geometry(np, pts, nc, cells, nf, fnodes, fcells, fnodes_r, cA, cdet, fn, fa, fv);
+    int64_t nd = 0;
+    for (int64_t f = 0; f < nf; ++f) dslot[f] = (fcells[2 * f + 1] < 0 && bc && bc[f] == 0) ? nd++ : -1;
+    double* Rc = (double*)calloc((size_t)nc * NV, sizeof(double));
+    double* Ri = (double*)calloc((size_t)nc * NV, sizeof(double));
+    double* Rb = (double*)calloc((size_t)nc * NV, sizeof(double));
+    const int am = threads > 1;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+
+    /* eval_cells: ExplicitConvection.cpp:144-198 */
+    if (parts & 1) {
+        double* vb = (double*)malloc(sizeof(double) * S.Q * N);
+        double* vg = (double*)malloc(sizeof(double) * S.Q * 3 * N);
+        for (int q = 0; q < S.Q; ++q) or_basis(order, S.qp[3 * q], S.qp[3 * q + 1], S.qp[3 * q + 2], vb + q * N, vg + q * 3 * N);
+#pragma omp parallel for schedule(dynamic) if (am)
+        for (int64_t e = 0; e < nc; ++e) {
+            const double* coef = U + e * NV;
+            for (int q = 0; q < S.Q; ++q) {
+                double Uq[5] = {0, 0, 0, 0, 0}, F[15];
+                for (int k = 0; k < N; ++k)
+                    for (int v = 0; v < 5; ++v) Uq[v] += vb[q * N + k] * coef[5 * k + v];
+                or_flux(Uq, gamma, F);
+                const double jw = S.qw[q] * cdet[e];
+                for (int j = 0; j < N; ++j) {
+                    double J[9], gp[3], fl[5];
+                    or_inverse3(cA + 9 * e, J);
+                    const double* gr = vg + (q * N + j) * 3;
+                    for (int d = 0; d < 3; ++d) gp[d] = J[3 * d] * gr[0] + J[3 * d + 1] * gr[1] + J[3 * d + 2] * gr[2];
+                    for (int v = 0; v < 5; ++v) fl[v] = F[3 * v] * gp[0] + F[3 * v + 1] * gp[1] + F[3 * v + 2] * gp[2];
+                    add_scaled(Rc + e * NV + 5 * j, fl, -jw, 5, am);
+                }
+            }
+        }
+        free(vb);
+        free(vg);
+    }
+
+    /* eval_internals (:201-272) and eval_boundarys (:21-128) */
+    if (parts & 6) {
+#pragma omp parallel for schedule(dynamic) if (am)
+        for (int64_t f = 0; f < nf; ++f) {
+            const int64_t c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
+            const int bnd = c1 < 0;
+            if (bnd && !(parts & 4)) continue;
+            if (!bnd && !(parts & 2)) continue;
+            if (bnd && (!bc || (bc[f] != 0 && bc[f] != 5))) continue;
+            double* phiL = (double*)malloc(sizeof(double) * N);
+            double* phiR = (double*)malloc(sizeof(double) * N);
+            const double jf = fa[f] * 2.0;
+            for (int g = 0; g < S.Qf; ++g) {
+                const double jw = S.fw[g] * jf;
+                double xiL[3], UL[5] = {0, 0, 0, 0, 0}, UR[5] = {0, 0, 0, 0, 0}, Fh[5], t[5];
+                face_to_cell(fv + 6 * f, S.fp[2 * g], S.fp[2 * g + 1], xiL);
+                or_basis(order, xiL[0], xiL[1], xiL[2], phiL, NULL);
+                for (int k = 0; k < N; ++k)
+                    for (int v = 0; v < 5; ++v) UL[v] += phiL[k] * U[c0 * NV + 5 * k + v];
+                if (!bnd) {
+                    double xiR[3];
+                    face_to_cell(fv + 6 * f + 3, S.fp[2 * g], S.fp[2 * g + 1], xiR);
+                    or_basis(order, xiR[0], xiR[1], xiR[2], phiR, NULL);
+                    for (int k = 0; k < N; ++k)
+                        for (int v = 0; v < 5; ++v) UR[v] += phiR[k] * U[c1 * NV + 5 * k + v];
+                } else if (bc[f] == 0) {
+                    const double* ud = UD + (dslot[f] * S.Qf + g) * 5;
+                    for (int v = 0; v < 5; ++v) UR[v] = ud[v] + (ud[v] - UL[v]);
+                } else {
+                    for (int v = 0; v < 5; ++v) UR[v] = UL[v];
+                    UR[3] = -UL[3];
+                }
+                num_flux(flux, UL, UR, fn + 3 * f, gamma, Fh);
+                for (int j = 0; j < N; ++j) {
+                    for (int v = 0; v < 5; ++v) t[v] = Fh[v] * phiL[j];
+                    add_scaled((bnd ? Rb : Ri) + c0 * NV + 5 * j, t, jw, 5, am);
+                    if (!bnd) {
+                        for (int v = 0; v < 5; ++v) t[v] = Fh[v] * phiR[j];
+                        add_scaled(Ri + c1 * NV + 5 * j, t, -jw, 5, am);
+                    }
+                }
+            }
+            free(phiL);
+            free(phiR);
+        }
+    }
+    for (int64_t i = 0; i < nc * NV; ++i) R[i] = Rc[i] + Ri[i] + Rb[i];
+    free(Rc);
+    free(Ri);
+    free(Rb);
+    free(cA);
+    free(cdet);
+    free(fn);
+    free(fa);
+    free(fv);
+    free(dslot);
+    return 0;
+}
+
+/* diagonal mass M[e*5N+5k+v] = sum_q phi_k^2 w_q |det| (VortexBatch.cpp:155-163) */
+int or_mass(int order, int64_t np, const double* pts, int64_t nc, const int64_t* cells, double* M) {
+    const double *qp, *qw;
+    const int Q = or_tet_rule(order, &qp, &qw), N = or_num_basis(order);
+    double* det = (double*)malloc(sizeof(double) * nc);
+    or_geometry(np, pts, nc, cells, 0, NULL, NULL, NULL, NULL, det, NULL, NULL, NULL);
+    double* phi = (double*)malloc(sizeof(double) * Q * N);
+    for (int q = 0; q < Q; ++q) or_basis(order, qp[3 * q], qp[3 * q + 1], qp[3 * q + 2], phi + q * N, NULL);
+    for (int64_t e = 0; e < nc; ++e)
+        for (int k = 0; k < N; ++k) {
+            double s = 0;
+            for (int q = 0; q < Q; ++q) s += phi[q * N + k] * phi[q * N + k] * qw[q] * det[e];
+            for (int v = 0; v < 5; ++v) M[e * 5 * N + 5 * k + v] = s;
+        }
+    free(det);
+    free(phi);
+    return 0;
+}
+
+/* n_steps of Shu-Osher SSP-RK3 on dU/dt = -M^{-1} R(U), in place on U (no Dirichlet faces) */
+int or_ssprk3(int order, int flux, double gamma, int64_t np, const double* pts, int64_t nc, const int64_t* cells,
+              int64_t nf, const int64_t* fnodes, const int64_t* fcells, const int64_t* fnodes_r, const uint8_t* bc,
+              double* U, double dt, int n_steps, int threads) {
+    const int64_t n = nc * 5 * or_num_basis(order);
+    double *M = malloc(sizeof(double) * n), *R = malloc(sizeof(double) * n), *U1 = malloc(sizeof(double) * n);
+    or_mass(order, np, pts, nc, cells, M);
+    static const double A[3] = {0.0, 0.75, 1.0 / 3.0}, B[3] = {1.0, 0.25, 2.0 / 3.0};
+    for (int s = 0; s < n_steps; ++s) {
+        for (int st = 0; st < 3; ++st) {
+            const double* src = st == 0 ? U : U1;
+            or_residual(order, flux, gamma, np, pts, nc, cells, nf, fnodes, fcells, fnodes_r, bc, NULL, src, 7, R,
+                        threads);
+            double* dst = st == 2 ? U : U1;
+            for (int64_t i = 0; i < n; ++i) {
+                double x = src[i] + dt * (-R[i] / M[i]);
+                dst[i] = st == 0 ? B[0] * x : A[st] * U[i] + B[st] * x;
+            }
+        }
+    }
+    free(M);
+    free(R);
+    free(U1);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ initial data */
+/* isentropic vortex of examples/Vortex/problem.h:17-110 with rho_inf = p_inf = 1;
+ * P = gamma, strength, x0, y0, u_inf, v_inf, Lx, Ly (periods for the nearest image; 0 = none) */
+void or_vortex_state(const double* P, double t, const double x[3], double U[5]) {
+    const double g = P[0], phi = P[1];
+    double dx = x[0] - (P[2] + P[4] * t), dy = x[1] - (P[3] + P[5] * t);
+    if (P[6] > 0) dx -= P[6] * floor(dx / P[6] + 0.5);
+    if (P[7] > 0) dy -= P[7] * floor(dy / P[7] + 0.5);
+    const double r2 = dx * dx + dy * dy;
+    const double slope = (phi * phi * (g - 1)) / (16 * M_PI * M_PI * g);
+    const double rho = pow(1 - slope * exp(2 * (1 - r2)), 1.0 / (g - 1));
+    const double u = P[4] - phi * dy / (2 * M_PI) * exp(1 - r2);
+    const double v = P[5] + phi * dx / (2 * M_PI) * exp(1 - r2);
+    const double p = pow(rho, g);
+    U[0] = rho;
+    U[1] = rho * u;
+    U[2] = rho * v;
+    U[3] = 0;
+    U[4] = rho * (p / rho / (g - 1) + 0.5 * (u * u + v * v));
+}
+
+/* func2coef of the vortex on every cell (diagonal projection, long double diagonal) */
+int or_project_vortex(int order, const double* P, double t, int64_t np, const double* pts, int64_t nc,
+                      const int64_t* cells, double* U) {
+    (void)np;
+    const double *qp, *qw;
+    const int Q = or_tet_rule(order, &qp, &qw), N = or_num_basis(order);
+    double* phi = (double*)malloc(sizeof(double) * Q * N);
+    long double* diag = (long double*)calloc(N, sizeof(long double));
+    for (int q = 0; q < Q; ++q) or_basis(order, qp[3 * q], qp[3 * q + 1], qp[3 * q + 2], phi + q * N, NULL);
+    for (int k = 0; k < N; ++k)
+        for (int q = 0; q < Q; ++q) diag[k] += (long double)phi[q * N + k] * phi[q * N + k] * qw[q];
+    for (int64_t e = 0; e < nc; ++e) {
+        const double *p0 = pts + 3 * cells[4 * e], *p1 = pts + 3 * cells[4 * e + 1], *p2 = pts + 3 * cells[4 * e + 2],
+                     *p3 = pts + 3 * cells[4 * e + 3];
+        for (int k = 0; k < N; ++k) {
+            double rhs[5] = {0, 0, 0, 0, 0};
+            for (int q = 0; q < Q; ++q) {
+                const double* xi = qp + 3 * q;
+                double x[3], Us[5];
+                for (int d = 0; d < 3; ++d)
+                    x[d] = p0[d] * (1 - xi[0] - xi[1] - xi[2]) + p1[d] * xi[0] + p2[d] * xi[1] + p3[d] * xi[2];
+                or_vortex_state(P, t, x, Us);
+                for (int v = 0; v < 5; ++v) rhs[v] += Us[v] * (phi[q * N + k] * qw[q]);
+            }
+            for (int v = 0; v < 5; ++v) U[e * 5 * N + 5 * k + v] = rhs[v] / (double)diag[k];
+        }
+    }
+    free(phi);
+    free(diag);
+    return 0;
+}
+
+/* threads the OpenMP runtime would use (for the baseline record) */
+int or_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
