@@ -115,6 +115,61 @@ SchemeTables build_tables(int order) {
     return T;
 }
 
+double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& PhiC, std::vector<uint8_t>& canon,
+                                   std::vector<int32_t>& perm) {
+    const int N = T.N, QF = T.Qf;
+    // canonical orders: for each vertex subset {x<y<z}: (x,y,z) then (x,z,y)
+    int canon_combo[8];
+    int n = 0;
+    for (int x = 0; x < 4; ++x)
+        for (int y = x + 1; y < 4; ++y)
+            for (int z = y + 1; z < 4; ++z) {
+                canon_combo[n++] = combo_index(x, y, z);
+                canon_combo[n++] = combo_index(x, z, y);
+            }
+    PhiC.assign(8 * QF * N, 0.0);
+    for (int c = 0; c < 8; ++c)
+        for (int g = 0; g < QF; ++g)
+            for (int k = 0; k < N; ++k) PhiC[(c * QF + g) * N + k] = T.Phi[((size_t)canon_combo[c] * QF + g) * N + k];
+    // point permutations: rotation m maps barycentrics (l0,l1,l2) to (l_m, l_m+1, l_m+2)
+    perm.assign(3 * QF, 0);
+    double worst_match = 0.0;
+    for (int m = 0; m < 3; ++m)
+        for (int g = 0; g < QF; ++g) {
+            const double u = T.fq_pts[2 * g], v = T.fq_pts[2 * g + 1];
+            const double lam[3] = {1 - u - v, u, v};
+            const double up = lam[(1 + m) % 3], vp = lam[(2 + m) % 3];
+            int best = -1;
+            double bd = 1e300;
+            for (int h = 0; h < QF; ++h) {
+                const double d = std::abs(T.fq_pts[2 * h] - up) + std::abs(T.fq_pts[2 * h + 1] - vp);
+                if (d < bd) bd = d, best = h;
+            }
+            perm[m * QF + g] = best;
+            worst_match = std::max(worst_match, bd);
+        }
+    if (worst_match > 1e-12) return INFINITY;  // the rule is not rotation invariant
+    canon.assign(24, 0);
+    double dev = 0.0;
+    for (int c = 0; c < 24; ++c) {
+        int a[3];
+        combo_vertices(c, a);
+        int m = 0;
+        for (int i = 1; i < 3; ++i)
+            if (a[i] < a[m]) m = i;
+        const int r0 = a[m], r1 = a[(m + 1) % 3], r2 = a[(m + 2) % 3];
+        const int cc = combo_index(r0, r1, r2);
+        int ci = -1;
+        for (int i = 0; i < 8; ++i)
+            if (canon_combo[i] == cc) ci = i;
+        canon[c] = (uint8_t)(ci * 3 + m);
+        for (int g = 0; g < QF; ++g)
+            for (int k = 0; k < N; ++k)
+                dev = std::max(dev, std::abs(T.Phi[((size_t)c * QF + g) * N + k] - PhiC[(ci * QF + perm[m * QF + g]) * N + k]));
+    }
+    return dev;
+}
+
 // ---------------------------------------------------------------- analytic fields
 bool analytic_state(int field, const double* P, int np, double t, const vector3f& x, double U[5]) {
     if (field == FVDG_FIELD_VORTEX) {

@@ -39,6 +39,13 @@ struct SchemeTables {
 };
 SchemeTables build_tables(int order);
 
+// Face tables reduced by the rotation symmetry of the face rule: 8 canonical
+// vertex orders (starting at the smallest reference vertex), per combo its
+// canonical index*3 + rotation, and the 3 point permutations of the rule.
+// Returns the largest deviation |Phi[c][g][k] - PhiC[canon][perm[rot][g]][k]|.
+double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& PhiC, std::vector<uint8_t>& canon,
+                                   std::vector<int32_t>& perm);
+
 // Map face point g of a face seen through `combo` to reference coordinates, with
 // the reference's operation order nc0*(1-u-v) + nc1*u + nc2*v.
 vector3f face_point_to_cell(int combo, double u, double v);

@@ -6,9 +6,11 @@
 //   R_faces  [L,j,v] += sum_g w_g 2A_f Fhat_v(U_L,U_R,n) phi_j(xi_L(g))                 (:201-272)
 //            [R,j,v] -= sum_g w_g 2A_f Fhat_v(U_L,U_R,n) phi_j(xi_R(g))
 //   boundary faces use a ghost state: Dirichlet 2 U_D - U_in, Pseudo3DZ (1,1,1,-1,1) U_in  (:21-128)
-// These kernels are the plain FP64 path: pointwise fluxes go through scratch
-// buffers (Fq: E*Q*15, Fh: F*Qf*5) and every contraction is a thread-private
-// loop, so each output has one fixed summation order (deterministic, no atomics).
+// The kernels in this file are the plain FP64 path ("simple"): pointwise fluxes go
+// through scratch buffers (Fq: E*Q*15, Fh: F*Qf*5) and every contraction is a
+// thread-private loop, so each output has one fixed summation order.  They serve
+// as the GPU cross-check of the tuned kernels (fvdg_kernels_fast.hpp) at sizes the
+// CPU oracle cannot reach.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,66 +41,93 @@ struct KernelArgs {
     double* Fh = nullptr;             // F*Qf*5
 };
 
-// element e, coefficient kv (= 5k+v) in the 64-element tiled layout
-__device__ __forceinline__ int64_t cidx(int64_t e, int kv, int NV) {
-    return ((e >> 6) * NV + kv) * 64 + (e & 63);
-}
+// element e, coefficient kv (= 5k+v): element-major, the reference's LongVector layout
+__device__ __forceinline__ int64_t cidx(int64_t e, int kv, int NV) { return e * NV + kv; }
 
 // ------------------------------------------------------------------ Euler algebra
-// Same formulas as src/DG/DG_Flux/EulerPhysicalFlux.cpp:7-18,46-55,70-89.
+// The formulas of src/DG/DG_Flux/EulerPhysicalFlux.cpp:7-18,46-55,70-89
+// (rho = max(U0, 1e-16), u = U1/rho, ..., p = (gamma-1)(rho e - rho |u|^2 / 2),
+// c = sqrt(gamma max(p / max(U0,1e-16), 1e-16))), evaluated once per state with
+// one reciprocal 1/rho instead of the reference's repeated divisions (each
+// quotient is then within one rounding of the reference's).
 __device__ __forceinline__ double pos_eps(double x) { return fmax(x, 1e-16); }
 
-__device__ __forceinline__ double euler_pressure(const double U[5], double gamma) {
-    const double rho = pos_eps(U[0]);
-    const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, e = U[4] / rho;
-    return (gamma - 1) * (rho * e - 0.5 * rho * (u * u + v * v + w * w));
+struct Prim {
+    double rho, ri, u, v, w, e, p;  // ri = 1/rho
+};
+
+__device__ __forceinline__ Prim euler_prim(const double U[5], double gamma) {
+    Prim q;
+    q.rho = pos_eps(U[0]);
+    const double ri = 1.0 / q.rho;
+    q.ri = ri;
+    q.u = U[1] * ri;
+    q.v = U[2] * ri;
+    q.w = U[3] * ri;
+    q.e = U[4] * ri;
+    q.p = (gamma - 1) * (q.rho * q.e - 0.5 * q.rho * (q.u * q.u + q.v * q.v + q.w * q.w));
+    return q;
 }
 
-__device__ __forceinline__ void euler_flux(const double U[5], double gamma, double F[5][3]) {
-    const double rho = pos_eps(U[0]);
-    const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, e = U[4] / rho;
-    const double p = euler_pressure(U, gamma);
+__device__ __forceinline__ double euler_pressure(const double U[5], double gamma) { return euler_prim(U, gamma).p; }
+
+__device__ __forceinline__ void euler_flux_prim(const Prim& q, double F[5][3]) {
+    const double rho = q.rho, u = q.u, v = q.v, w = q.w, p = q.p;
+    const double re = rho * q.e + p;
     F[0][0] = rho * u;         F[0][1] = rho * v;         F[0][2] = rho * w;
     F[1][0] = rho * u * u + p; F[1][1] = rho * v * u;     F[1][2] = rho * w * u;
     F[2][0] = rho * u * v;     F[2][1] = rho * v * v + p; F[2][2] = rho * w * v;
     F[3][0] = rho * u * w;     F[3][1] = rho * v * w;     F[3][2] = rho * w * w + p;
-    F[4][0] = u * (rho * e + p); F[4][1] = v * (rho * e + p); F[4][2] = w * (rho * e + p);
+    F[4][0] = u * re;          F[4][1] = v * re;          F[4][2] = w * re;
+}
+
+__device__ __forceinline__ void euler_flux(const double U[5], double gamma, double F[5][3]) {
+    euler_flux_prim(euler_prim(U, gamma), F);
+}
+
+// normal flux F(U) n from primitives
+__device__ __forceinline__ void euler_flux_n(const Prim& q, const double n[3], double out[5]) {
+    const double un = q.u * n[0] + q.v * n[1] + q.w * n[2];
+    const double mn = q.rho * un;
+    out[0] = mn;
+    out[1] = mn * q.u + q.p * n[0];
+    out[2] = mn * q.v + q.p * n[1];
+    out[3] = mn * q.w + q.p * n[2];
+    out[4] = un * (q.rho * q.e + q.p);
+}
+
+// |u| + c; c uses p / max(U0, 1e-16) = p / rho
+__device__ __forceinline__ double euler_wave_speed_prim(const Prim& q, double gamma) {
+    const double c = sqrt(gamma * pos_eps(q.p * q.ri));
+    return sqrt(q.u * q.u + q.v * q.v + q.w * q.w) + c;
 }
 
 __device__ __forceinline__ double euler_wave_speed(const double U[5], double gamma) {
-    const double rho = pos_eps(U[0]);
-    const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho;
-    const double c = sqrt(gamma * pos_eps(euler_pressure(U, gamma) / pos_eps(U[0])));
-    return sqrt(u * u + v * v + w * w) + c;
+    return euler_wave_speed_prim(euler_prim(U, gamma), gamma);
 }
 
 // Lax-Friedrichs: 0.5 (F(U_L) n + F(U_R) n + lambda (U_L - U_R)), lambda = max(|u|+c)
 __device__ __forceinline__ void flux_lf(const double UL[5], const double UR[5], const double n[3], double gamma,
                                         double out[5]) {
-    double FL[5][3], FR[5][3];
-    euler_flux(UL, gamma, FL);
-    euler_flux(UR, gamma, FR);
-    const double lam = fmax(euler_wave_speed(UL, gamma), euler_wave_speed(UR, gamma));
-    for (int v = 0; v < 5; ++v) {
-        const double fl = FL[v][0] * n[0] + FL[v][1] * n[1] + FL[v][2] * n[2];
-        const double fr = FR[v][0] * n[0] + FR[v][1] * n[1] + FR[v][2] * n[2];
-        out[v] = 0.5 * (fl + fr + lam * (UL[v] - UR[v]));
-    }
+    const Prim L = euler_prim(UL, gamma), R = euler_prim(UR, gamma);
+    double fl[5], fr[5];
+    euler_flux_n(L, n, fl);
+    euler_flux_n(R, n, fr);
+    const double lam = fmax(euler_wave_speed_prim(L, gamma), euler_wave_speed_prim(R, gamma));
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fl[v] + fr[v] + lam * (UL[v] - UR[v]));
 }
 
 // Roe (1981) flux with Harten's entropy fix on the acoustic waves, delta = 0.1 * a_roe:
 //   Fhat = 0.5 (F_L n + F_R n) - 0.5 sum_k |lambda_k| alpha_k r_k
 __device__ __forceinline__ void flux_roe(const double UL[5], const double UR[5], const double n[3], double gamma,
                                          double out[5]) {
-    double FL[5][3], FR[5][3];
-    euler_flux(UL, gamma, FL);
-    euler_flux(UR, gamma, FR);
-    const double rL = pos_eps(UL[0]), rR = pos_eps(UR[0]);
-    const double pL = euler_pressure(UL, gamma), pR = euler_pressure(UR, gamma);
-    const double uL[3] = {UL[1] / rL, UL[2] / rL, UL[3] / rL};
-    const double uR[3] = {UR[1] / rR, UR[2] / rR, UR[3] / rR};
-    const double HL = (UL[4] + pL) / rL, HR = (UR[4] + pR) / rR;
-    const double sL = sqrt(rL), sR = sqrt(rR), inv = 1.0 / (sL + sR);
+    const Prim L = euler_prim(UL, gamma), R = euler_prim(UR, gamma);
+    double fl[5], fr[5];
+    euler_flux_n(L, n, fl);
+    euler_flux_n(R, n, fr);
+    const double uL[3] = {L.u, L.v, L.w}, uR[3] = {R.u, R.v, R.w};
+    const double HL = (UL[4] + L.p) * L.ri, HR = (UR[4] + R.p) * R.ri;
+    const double sL = sqrt(L.rho), sR = sqrt(R.rho), inv = 1.0 / (sL + sR);
     double ut[3], q2 = 0, qn = 0;
     for (int d = 0; d < 3; ++d) {
         ut[d] = (sL * uL[d] + sR * uR[d]) * inv;
@@ -107,9 +136,9 @@ __device__ __forceinline__ void flux_roe(const double UL[5], const double UR[5],
     }
     const double Ht = (sL * HL + sR * HR) * inv;
     const double a2 = fmax((gamma - 1) * (Ht - 0.5 * q2), 1e-16);
-    const double a = sqrt(a2);
+    const double a = sqrt(a2), ia2 = 1.0 / a2;
     const double rt = sL * sR;
-    const double dr = rR - rL, dp = pR - pL;
+    const double dr = R.rho - L.rho, dp = R.p - L.p;
     double du[3], dqn = 0;
     for (int d = 0; d < 3; ++d) {
         du[d] = uR[d] - uL[d];
@@ -120,25 +149,21 @@ __device__ __forceinline__ void flux_roe(const double UL[5], const double UR[5],
         dut[d] = du[d] - dqn * n[d];
         ut_dut += ut[d] * dut[d];
     }
-    const double a1 = (dp - rt * a * dqn) / (2 * a2);
-    const double a5 = (dp + rt * a * dqn) / (2 * a2);
-    const double a_2 = dr - dp / a2;
+    const double a1 = (dp - rt * a * dqn) * (0.5 * ia2);
+    const double a5 = (dp + rt * a * dqn) * (0.5 * ia2);
+    const double a_2 = dr - dp * ia2;
     const double delta = 0.1 * a;
     auto fix = [delta](double lam) {
         const double al = fabs(lam);
         return al < delta ? (lam * lam + delta * delta) / (2 * delta) : al;
     };
     const double l1 = fix(qn - a), l2 = fabs(qn), l5 = fix(qn + a);
+    const double w1 = l1 * a1, w2 = l2 * a_2, w5 = l5 * a5, ws = l2 * rt;
     double d[5];
-    d[0] = l1 * a1 + l2 * a_2 + l5 * a5;
-    for (int k = 0; k < 3; ++k)
-        d[1 + k] = l1 * a1 * (ut[k] - a * n[k]) + l2 * a_2 * ut[k] + l2 * rt * dut[k] + l5 * a5 * (ut[k] + a * n[k]);
-    d[4] = l1 * a1 * (Ht - a * qn) + l2 * a_2 * 0.5 * q2 + l2 * rt * ut_dut + l5 * a5 * (Ht + a * qn);
-    for (int v = 0; v < 5; ++v) {
-        const double fl = FL[v][0] * n[0] + FL[v][1] * n[1] + FL[v][2] * n[2];
-        const double fr = FR[v][0] * n[0] + FR[v][1] * n[1] + FR[v][2] * n[2];
-        out[v] = 0.5 * (fl + fr) - 0.5 * d[v];
-    }
+    d[0] = w1 + w2 + w5;
+    for (int k = 0; k < 3; ++k) d[1 + k] = w1 * (ut[k] - a * n[k]) + w2 * ut[k] + ws * dut[k] + w5 * (ut[k] + a * n[k]);
+    d[4] = w1 * (Ht - a * qn) + w2 * 0.5 * q2 + ws * ut_dut + w5 * (Ht + a * qn);
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * d[v];
 }
 
 __device__ __forceinline__ void num_flux(int kind, const double UL[5], const double UR[5], const double n[3],
@@ -147,22 +172,6 @@ __device__ __forceinline__ void num_flux(int kind, const double UL[5], const dou
         flux_roe(UL, UR, n, gamma, out);
     else
         flux_lf(UL, UR, n, gamma, out);
-}
-
-// ------------------------------------------------------------------ layout transposes
-__global__ void k_aos_to_tiled(const double* __restrict__ aos, double* __restrict__ tiled, int64_t E, int NV) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E * NV) return;
-    const int64_t e = i / NV;
-    const int kv = (int)(i - e * NV);
-    tiled[cidx(e, kv, NV)] = aos[i];
-}
-__global__ void k_tiled_to_aos(const double* __restrict__ tiled, double* __restrict__ aos, int64_t E, int NV) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E * NV) return;
-    const int64_t e = i / NV;
-    const int kv = (int)(i - e * NV);
-    aos[i] = tiled[cidx(e, kv, NV)];
 }
 
 // ------------------------------------------------------------------ volume term
@@ -288,22 +297,15 @@ __global__ void k_face_test(KernelArgs a) {
 }
 
 // ------------------------------------------------------------------ SSP-RK stage update
-// out = a0*U0 + b0*(Us + dt * (-R / M)),  M = |det A_e| * mass_hat[k]  (padding lanes -> 0)
+// out = a0*U0 + b0*(Us + dt * (-R / M)),  M = |det A_e| * mass_hat[k]
 __global__ void k_rk_update(const double* __restrict__ U0, const double* __restrict__ Us,
                             const double* __restrict__ R, double* __restrict__ out, const double* __restrict__ cgeo,
                             const double* __restrict__ mhat, int64_t E, int N, double a0, double b0, double dt) {
     const int NV = 5 * N;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t Epad = (E + 63) / 64 * 64;
-    if (i >= Epad * NV) return;
-    const int64_t tile = i / (NV * 64);
-    const int64_t rem = i - tile * NV * 64;
-    const int kv = (int)(rem >> 6);
-    const int64_t e = tile * 64 + (rem & 63);
-    if (e >= E) {
-        out[i] = 0.0;
-        return;
-    }
+    if (i >= E * NV) return;
+    const int64_t e = i / NV;
+    const int kv = (int)(i - e * NV);
     const double M = cgeo[e * 10 + 9] * mhat[kv / 5];
     const double L = -R[i] / M;
     const double s = Us[i] + dt * L;

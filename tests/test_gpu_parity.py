@@ -132,3 +132,38 @@ def test_residual_medium_mesh(solver, oracle):
     Rg = solver.residual()
     Ro = oracle.residual(mesh.export(), 3, U, flux="roe", threads=8)
     assert rel_err(Rg, Ro) < TOL
+
+
+def _simple_solver():
+    """A second context running the plain per-point kernels (FVDG_KERNELS=simple)."""
+    import os
+
+    old = os.environ.get("FVDG_KERNELS")
+    os.environ["FVDG_KERNELS"] = "simple"
+    try:
+        return F.Solver(0)
+    finally:
+        if old is None:
+            del os.environ["FVDG_KERNELS"]
+        else:
+            os.environ["FVDG_KERNELS"] = old
+
+
+@pytest.mark.parametrize("order,flux,n", [(3, "roe", 32), (4, "lf", 16), (1, "lf", 24)])
+def test_tuned_kernels_match_simple_kernels_full_size(solver, order, flux, n):
+    """At sizes beyond the oracle's reach, the tuned kernels (MFMA volume, tiled
+    owner-computes faces, fused update) against the plain per-point kernels."""
+    print(f"[case] tuned vs simple n={n} P{order} {flux}", flush=True)
+    mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=60)
+    U = perturbed_vortex(mesh, order, seed=3, amp=1e-3)
+    simple = _simple_solver()
+    try:
+        for s in (solver, simple):
+            s.set_scheme(order, flux=flux).set_mesh(mesh)
+            s.upload(U)
+        assert rel_err(solver.residual(), simple.residual()) < TOL
+        for s in (solver, simple):
+            s.ssprk3(2e-3, 2)
+        assert rel_err(solver.download(), simple.download()) < 1e-13
+    finally:
+        simple.close()
