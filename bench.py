@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="vortex_p3_196k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="before the run, copy MIB MiB device-to-device once (a plain copy kernel of known "
+                         "size, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path is a design (element partition + halo "
@@ -87,6 +90,15 @@ def main():
     import numpy as np
 
     import fve_dg_amd as F
+
+    if args.calibrate_copy > 0:
+        import torch
+
+        src = torch.ones(args.calibrate_copy * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     n, order, flux, jitter = WORKLOADS[args.workload]
     t_setup = time.perf_counter()
