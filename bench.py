@@ -34,14 +34,22 @@ WORKLOADS = {
 }
 VORTEX = [1.4, 5.0, 5.0, 5.0, 1.0, 1.0, 10.0, 10.0]
 
+# HBM bytes per launch at the default workload from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (factors from a 1 GiB device copy); see profiles/.  Filled in per kernel when measured.
+TRAFFIC = {}
+
 
 def algorithmic(order, Q, Qf, N):
-    """Per element and stage: flops of the volume and surface kernels, bytes of the update kernel."""
-    vol_flops = 40 * Q * N + 145 * Q            # interpolation 10QN, test 30QN, pointwise flux ~145Q
-    surf_flops = 80 * Qf * N + 410 * Qf         # 2 faces: traces 20QfN, test 20QfN, Riemann ~205Qf each
-    upd_bytes = 4 * 8 * 5 * N + 8               # read U^n, U^s, R; write U; |det|
-    stage_bytes = 8 * 5 * N * (2 + 2 / 3) + 168  # compulsory traffic of one fused stage
-    return vol_flops, surf_flops, upd_bytes, stage_bytes
+    """Algorithmic work per element and stage (FMA = 2 flops):
+    volume kernel      40QN + 145Q   (interpolation 10QN, test 30QN, pointwise flux ~145Q)
+    face-flux kernel   40QfN + 410Qf (2 faces x [2 traces 10QfN each + Riemann ~205Qf])
+    face-test kernel   40QfN + 25N   (4 face sides x 10QfN test sums + the stage update)
+    bytes of one stage 8*5N*(2 + 2/3) + 168 (state in/out, U^n in 2 of 3 stages, geometry)."""
+    vol_flops = 40 * Q * N + 145 * Q
+    flux_flops = 40 * Qf * N + 410 * Qf
+    test_flops = 40 * Qf * N + 25 * N
+    stage_bytes = 8 * 5 * N * (2 + 2 / 3) + 168
+    return vol_flops, flux_flops, test_flops, stage_bytes
 
 
 def cpu_baseline(workload_n, order, flux, seconds_target=15.0):
@@ -126,20 +134,24 @@ def main():
     dof = 5 * N * E
     value = dof / (stage_ms * 1e-3)
 
-    vol_f, surf_f, upd_b, stage_b = algorithmic(order, Q, Qf, N)
+    vol_f, flux_f, test_f, stage_b = algorithmic(order, Q, Qf, N)
+
+    def rate(work, t_ms):
+        return work * E / (t_ms * 1e-3) if t_ms > 0 else 0.0
+
     kernels = {
-        "volume": {"ms": ms[1], "achieved": vol_f * E / (ms[1] * 1e-3), "peak": PEAK_FP64, "bound": "mfma",
-                   "unit": "TFLOP/s", "scale": 1e12},
-        "surface": {"ms": ms[2], "achieved": surf_f * E / (ms[2] * 1e-3), "peak": PEAK_FP64, "bound": "mfma",
-                    "unit": "TFLOP/s", "scale": 1e12},
-        "update": {"ms": ms[3], "achieved": upd_b * E / (ms[3] * 1e-3), "peak": PEAK_HBM, "bound": "hbm",
-                   "unit": "GB/s", "scale": 1e9},
+        "volume (k_vol_mfma)": {"ms": ms[1], "achieved": rate(vol_f, ms[1]), "peak": PEAK_FP64, "bound": "mfma",
+                                "unit": "TFLOP/s", "scale": 1e12},
+        "face flux (k_face_flux_lds)": {"ms": ms[2], "achieved": rate(flux_f, ms[2]), "peak": PEAK_FP64,
+                                        "bound": "mfma", "unit": "TFLOP/s", "scale": 1e12},
+        "face test + update (k_face_test_lds)": {"ms": ms[3], "achieved": rate(test_f, ms[3]), "peak": PEAK_FP64,
+                                                 "bound": "mfma", "unit": "TFLOP/s", "scale": 1e12},
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     k = kernels[dom]
     roofline = {"bound": k["bound"], "achieved": round(k["achieved"] / k["scale"], 3),
                 "peak": round(k["peak"] / k["scale"], 1), "unit": k["unit"],
-                "frac": round(k["achieved"] / k["peak"], 4), "traffic": None,
+                "frac": round(k["achieved"] / k["peak"], 4), "traffic": TRAFFIC.get(dom),
                 "kernel": dom, "kernel_ms": round(k["ms"], 4),
                 "all_kernels": {name: {"ms": round(v["ms"], 4), "achieved": round(v["achieved"] / v["scale"], 3),
                                        "unit": v["unit"], "frac": round(v["achieved"] / v["peak"], 4)}

@@ -14,21 +14,38 @@
 //   staged once through LDS with full-width coalesced loads, and the result goes
 //   back the same way.  Accumulators live in AGPRs.
 //
-// k_surf_tile<ORDER,FUSE>: face terms, owner-computes (each element evaluates its
-//   own four faces, so no two blocks write the same element: no atomics, fixed
-//   summation order).  A block owns 8 elements (4 at P4):
-//     0. face metadata of the 32 (element, local face) slots
-//     1. stage the 8 own coefficient blocks and the 32 neighbour blocks in LDS
-//        (contiguous 5N-double copies)
-//     2. lanes = (element, face, 3 face points): both traces, ghost states,
-//        LF/Roe flux, scaled by +-w_g 2A_f, into LDS
-//     3. lanes = (element, test function j): sum over 4 faces x Qf points
-//   With FUSE the SSP-RK stage update (inverse diagonal mass + Shu-Osher
-//   combination) is applied right there and the residual never goes to memory.
+// Faces (no atomics anywhere; every output has one fixed summation order):
+//   k_face_flux_lds + k_face_test_lds  two passes (default): each face's traces and
+//     Riemann flux once, then the test functions per element;
+//   k_surf_lds                          one pass, owner-computes: each element
+//     evaluates its own four faces (both traces and the flux are computed by both
+//     neighbours).
+//   Both finish with the SSP-RK stage update (inverse diagonal mass + Shu-Osher
+//   combination) when used inside the time stepper, so the total residual never goes
+//   to memory.
 #pragma once
 #include "fvdg_kernels.hpp"
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Cooperative global -> LDS copy of TOTAL doubles by NT threads: every load of the
+// thread is issued before the first LDS store, so the block pays one memory
+// latency per staging round instead of one per element copied.
+template <int TOTAL, int NT, typename Src, typename Dst>
+__device__ __forceinline__ void stage_to_lds(int t, Src src, Dst dst) {
+    constexpr int PF = (TOTAL + NT - 1) / NT;
+    double tmp[PF];
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {
+        const int i = t + m * NT;
+        tmp[m] = (i < TOTAL) ? src(i) : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {
+        const int i = t + m * NT;
+        if (i < TOTAL) dst(i, tmp[m]);
+    }
+}
 
 template <int ORDER>
 struct Ord;
@@ -70,10 +87,9 @@ __global__ __launch_bounds__(64) void k_vol_mfma(VolArgs a) {
     const int64_t base = e0 * NV, limit = a.E * NV;
 
     // 1. stage the tile (16*NV contiguous doubles) in LDS
-    for (int i = l; i < 16 * NV; i += 64) {
-        const int ee = i / NV, kv = i - ee * NV;
-        sC[ee * STR + kv] = (base + i < limit) ? a.U[base + i] : 0.0;
-    }
+    stage_to_lds<16 * NV, 64>(
+        l, [&](int i) { return (base + i < limit) ? a.U[base + i] : 0.0; },
+        [&](int i, double x) { sC[(i / NV) * STR + (i % NV)] = x; });
     __syncthreads();
 
     // 2. geometry of this lane's element (zeros past the end)
@@ -188,178 +204,9 @@ struct SurfArgs {
     double a0, b0, dt;
 };
 
-// elements per block: 8 (256 threads) up to P3, 4 (128 threads) at P4 to keep the
-// LDS footprint (own + 4 neighbour coefficient blocks + face fluxes) near 45-65 KB
-template <int ORDER>
-struct SurfShape {
-    static constexpr int TE = Ord<ORDER>::N > 20 ? 4 : 8;
-    static constexpr int THREADS = TE * 32;
-};
-
-template <int ORDER, bool FUSE>
-__global__ __launch_bounds__(SurfShape<ORDER>::THREADS) void k_surf_tile(SurfArgs a) {
-    constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
-    constexpr int TE = SurfShape<ORDER>::TE, NT = SurfShape<ORDER>::THREADS, GPT = (QF + 7) / 8;
-    __shared__ double sSelf[TE * NV];
-    __shared__ double sNbr[TE * 4 * NV];
-    __shared__ double sFh[TE * 4 * QF * 5];
-    __shared__ int sFace[TE * 4], sNb[TE * 4], sCs[TE * 4], sCn[TE * 4], sKind[TE * 4];
-    __shared__ double sScale[TE * 4];
-
-    const int t = threadIdx.x;
-    const int64_t e_base = (int64_t)blockIdx.x * TE;
-    const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
-
-    // 0. face metadata: kind 0 = skip, 1 = interior, 2 = boundary ghost
-    if (t < TE * 4) {
-        const int64_t e = e_base + t / 4;
-        int kind = 0, f = 0, nb = -1, cs = 0, cn = 0;
-        double scale = 0.0;
-        if (e < a.E) {
-            const int32_t slot = a.cfaces[e * 4 + (t & 3)];
-            f = slot >> 1;
-            const int side = slot & 1;
-            const int32_t other = a.fcell[2 * f + 1 - side];
-            const bool bnd = a.fcell[2 * f + 1] < 0;
-            cs = a.fcombo[2 * f + side];
-            cn = a.fcombo[2 * f + 1 - side];
-            if (!bnd && do_int) {
-                kind = 1;
-                nb = other;
-            } else if (bnd && do_bnd && (a.fbc[f] == FVDG_BC_DIRICHLET || a.fbc[f] == FVDG_BC_PSEUDO3DZ)) {
-                kind = 2;
-            }
-            scale = (side == 0 ? 1.0 : -1.0) * a.fgeo[4 * f + 3];
-        }
-        sFace[t] = f;
-        sNb[t] = nb;
-        sCs[t] = cs;
-        sCn[t] = cn;
-        sKind[t] = kind;
-        sScale[t] = scale;
-    }
-    // 1. own coefficient blocks (contiguous)
-    for (int i = t; i < TE * NV; i += NT) sSelf[i] = (e_base * NV + i < a.E * NV) ? a.U[e_base * NV + i] : 0.0;
-    __syncthreads();
-    //    neighbour blocks
-    for (int i = t; i < TE * 4 * NV; i += NT) {
-        const int s = i / NV, kv = i - s * NV;
-        const int nb = sNb[s];
-        sNbr[i] = nb >= 0 ? a.U[(int64_t)nb * NV + kv] : 0.0;
-    }
-    __syncthreads();
-
-    // 2. traces, ghost states and numerical flux at (element, face, GPT points)
-    {
-        const int s = t / 8;  // slot = element*4 + face
-        const int gg = t & 7;
-        const int kind = sKind[s];
-        const int side_is0 = sScale[s] >= 0.0;
-        double Us[GPT][5], Un[GPT][5];
-#pragma unroll
-        for (int i = 0; i < GPT; ++i)
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Us[i][v] = Un[i][v] = 0.0;
-        if (kind != 0) {
-            const double* Ps = a.PhiT + (int64_t)sCs[s] * N * QF;
-            const double* Pn = a.PhiT + (int64_t)sCn[s] * N * QF;
-            const double* cs = sSelf + (s >> 2) * NV;
-            const double* cn = sNbr + s * NV;
-#pragma unroll 4
-            for (int k = 0; k < N; ++k) {
-                double c[5], d[5];
-#pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    c[v] = cs[5 * k + v];
-                    d[v] = cn[5 * k + v];
-                }
-#pragma unroll
-                for (int i = 0; i < GPT; ++i) {
-                    const int g = gg * GPT + i;
-                    if (g < QF) {
-                        const double ps = Ps[k * QF + g];
-                        const double pn = Pn[k * QF + g];
-#pragma unroll
-                        for (int v = 0; v < 5; ++v) {
-                            Us[i][v] += ps * c[v];
-                            Un[i][v] += pn * d[v];
-                        }
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < GPT; ++i) {
-            const int g = gg * GPT + i;
-            if (g >= QF) continue;
-            double out[5] = {0, 0, 0, 0, 0};
-            if (kind != 0) {
-                const int f = sFace[s];
-                const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
-                double Ug[5];
-                if (kind == 2) {
-                    if (a.fbc[f] == FVDG_BC_DIRICHLET) {
-                        const double* ud = a.UD + ((int64_t)a.fdir[f] * QF + g) * 5;
-#pragma unroll
-                        for (int v = 0; v < 5; ++v) Ug[v] = ud[v] + (ud[v] - Us[i][v]);
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < 5; ++v) Ug[v] = Us[i][v];
-                        Ug[3] = -Us[i][3];
-                    }
-                    num_flux(a.flux, Us[i], Ug, n, a.gamma, out);
-                } else if (side_is0) {
-                    num_flux(a.flux, Us[i], Un[i], n, a.gamma, out);
-                } else {
-                    num_flux(a.flux, Un[i], Us[i], n, a.gamma, out);
-                }
-                const double sc = a.wf[g] * sScale[s];
-#pragma unroll
-                for (int v = 0; v < 5; ++v) out[v] *= sc;
-            }
-#pragma unroll
-            for (int v = 0; v < 5; ++v) sFh[(s * QF + g) * 5 + v] = out[v];
-        }
-    }
-    __syncthreads();
-
-    // 3. test functions: r[j][v] = sum_faces sum_g phi_j(xi(g)) * Fh_scaled(g)
-    for (int item = t; item < TE * N; item += NT) {
-        const int el = item / N, j = item - el * N;
-        const int64_t e = e_base + el;
-        if (e >= a.E) continue;
-        double r[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-        for (int lf = 0; lf < 4; ++lf) {
-            const int s = el * 4 + lf;
-            const double* P = a.Phi + (int64_t)sCs[s] * QF * N + j;
-            const double* fh = sFh + s * QF * 5;
-#pragma unroll 4
-            for (int g = 0; g < QF; ++g) {
-                const double p = P[g * N];
-#pragma unroll
-                for (int v = 0; v < 5; ++v) r[v] += p * fh[g * 5 + v];
-            }
-        }
-        const int64_t i0 = e * NV + 5 * j;
-        if (FUSE) {
-            const double M = a.cgeo[e * 10 + 9] * a.mhat[j];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                const double Rt = a.R[i0 + v] + r[v];
-                const double s = a.U[i0 + v] + a.dt * (-Rt / M);
-                a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * s : a.a0 * a.U0[i0 + v] + a.b0 * s;
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) a.R[i0 + v] = a.accumulate ? a.R[i0 + v] + r[v] : r[v];
-        }
-    }
-}
-
 // ------------------------------------------------------------------ surface, persistent
-// k_surf_lds<ORDER,FUSE>: same mathematics as k_surf_tile, organised so that no
-// inner loop touches global memory:
+// k_surf_lds<ORDER,FUSE>: owner-computes faces, organised so that no inner loop
+// touches global memory:
 //   * the face tables live in LDS for the whole kernel.  The face rules are
 //     invariant under cyclic rotation of the face vertices, so the 24
 //     (local face x vertex order) tables are 8 canonical ones (vertex order
@@ -438,7 +285,9 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
             sKind[t] = kind;
             sScale[t] = scale;
         }
-        for (int i = t; i < TE * NV; i += NT) sSelf[i] = (e_base * NV + i < a.E * NV) ? a.U[e_base * NV + i] : 0.0;
+        stage_to_lds<TE * NV, NT>(
+            t, [&](int i) { return (e_base * NV + i < a.E * NV) ? a.U[e_base * NV + i] : 0.0; },
+            [&](int i, double x) { sSelf[i] = x; });
         double r[ITEMS][5];
 #pragma unroll
         for (int it = 0; it < ITEMS; ++it)
@@ -448,11 +297,13 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
 #pragma unroll 1
         for (int lf = 0; lf < 4; ++lf) {
             __syncthreads();  // metadata ready / previous round's readers done
-            for (int i = t; i < TE * NV; i += NT) {
-                const int el = i / NV, kv = i - el * NV;
-                const int nb = sNb[el * 4 + lf];
-                sNbr[i] = nb >= 0 ? a.U[(int64_t)nb * NV + kv] : 0.0;
-            }
+            stage_to_lds<TE * NV, NT>(
+                t,
+                [&](int i) {
+                    const int nb = sNb[(i / NV) * 4 + lf];
+                    return nb >= 0 ? a.U[(int64_t)nb * NV + (i % NV)] : 0.0;
+                },
+                [&](int i, double x) { sNbr[i] = x; });
             __syncthreads();
             // traces + numerical flux, GP points per lane
             if (t < TE * NPG) {
@@ -568,6 +419,261 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
             } else {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) a.R[i0 + v] = a.accumulate ? a.R[i0 + v] + r[it][v] : r[it][v];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ faces in two passes
+// k_face_flux_lds<ORDER>: every face once (face-based, no duplicated work): lanes =
+//   (face, 2 face points); both traces from LDS-staged coefficient blocks and the
+//   canonical LDS tables, ghost state on boundary faces, LF/Roe flux, scaled by
+//   w_g * 2A_f and written to Fh[f][g][v] (contiguous per face).
+// k_face_test_lds<ORDER,FUSE>: lanes = (element, test function j); each local face's
+//   Fh of the tile's elements is staged in LDS with the side's sign, then
+//   sum_faces sum_g phi_j(xi_side(g)) Fh[g] is added to R or fed to the SSP-RK stage.
+struct FaceFluxArgs {
+    int64_t F;
+    double gamma;
+    int flux;
+    const double* U;
+    const int32_t* fcell;
+    const uint8_t* fcombo;
+    const double* fgeo;
+    const uint8_t* fbc;
+    const int32_t* fdir;
+    const double* UD;
+    const double* wf;
+    const double* PhiC;
+    const uint8_t* canon;
+    const int32_t* perm;
+    double* Fh;  // F*QF*5
+    int64_t ntiles;
+};
+
+template <int ORDER>
+struct FaceFluxShape {
+    static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF;
+    static constexpr int GP = 2, LPF = (QF + GP - 1) / GP;  // lanes per face
+    static constexpr int THREADS = 192;
+    static constexpr int FB = THREADS / LPF;                // faces per tile
+};
+
+template <int ORDER>
+__global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs a) {
+    using SH = FaceFluxShape<ORDER>;
+    constexpr int N = SH::N, QF = SH::QF, NV = 5 * N, GP = SH::GP, LPF = SH::LPF, FB = SH::FB;
+    constexpr int NT = SH::THREADS, NS = N | 1;  // odd table row stride: spreads LDS banks
+    __shared__ double sPhi[8 * QF * NS];
+    __shared__ double sC[2 * FB * NV];
+    __shared__ int sPerm[3 * QF];
+    __shared__ int sCanon[24];
+    __shared__ int sCell[2 * FB], sCb[2 * FB], sKind[FB];
+
+    const int t = threadIdx.x;
+    for (int i = t; i < 8 * QF * N; i += NT) sPhi[(i / N) * NS + (i % N)] = a.PhiC[i];
+    for (int i = t; i < 3 * QF; i += NT) sPerm[i] = a.perm[i];
+    if (t < 24) sCanon[t] = a.canon[t];
+
+    for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const int64_t f0 = tile * FB;
+        __syncthreads();
+        if (t < FB) {
+            const int64_t f = f0 + t;
+            int kind = 0, L = -1, R = -1, cL = 0, cR = 0;
+            if (f < a.F) {
+                L = a.fcell[2 * f];
+                R = a.fcell[2 * f + 1];
+                cL = sCanon[a.fcombo[2 * f]];
+                cR = sCanon[a.fcombo[2 * f + 1]];
+                if (R >= 0)
+                    kind = 1;
+                else if (a.fbc[f] == FVDG_BC_DIRICHLET || a.fbc[f] == FVDG_BC_PSEUDO3DZ)
+                    kind = 2;
+            }
+            sCell[2 * t] = (kind != 0) ? L : -1;
+            sCell[2 * t + 1] = (kind == 1) ? R : -1;
+            sCb[2 * t] = cL;
+            sCb[2 * t + 1] = cR;
+            sKind[t] = kind;
+        }
+        __syncthreads();
+        stage_to_lds<2 * FB * NV, NT>(
+            t,
+            [&](int i) {
+                const int c = sCell[i / NV];
+                return c >= 0 ? a.U[(int64_t)c * NV + (i % NV)] : 0.0;
+            },
+            [&](int i, double x) { sC[i] = x; });
+        __syncthreads();
+        if (t < FB * LPF) {
+            const int fl = t / LPF, pg = t - fl * LPF;
+            const int64_t f = f0 + fl;
+            const int kind = sKind[fl];
+            if (f < a.F) {
+                double UL[GP][5], UR[GP][5];
+                int gl[GP], gr[GP];
+                const int cL = sCb[2 * fl], cR = sCb[2 * fl + 1];
+                const double* PL = sPhi + (cL / 3) * QF * NS;
+                const double* PR = sPhi + (cR / 3) * QF * NS;
+#pragma unroll
+                for (int i = 0; i < GP; ++i) {
+                    const int g = min(pg * GP + i, QF - 1);
+                    gl[i] = sPerm[(cL % 3) * QF + g];
+                    gr[i] = sPerm[(cR % 3) * QF + g];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) UL[i][v] = UR[i][v] = 0.0;
+                }
+                if (kind != 0) {
+                    const double* c0 = sC + (2 * fl) * NV;
+                    const double* c1 = sC + (2 * fl + 1) * NV;
+#pragma unroll 4
+                    for (int k = 0; k < N; ++k) {
+                        double c[5], d[5];
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) {
+                            c[v] = c0[5 * k + v];
+                            d[v] = c1[5 * k + v];
+                        }
+#pragma unroll
+                        for (int i = 0; i < GP; ++i) {
+                            const double pl = PL[gl[i] * NS + k], pr = PR[gr[i] * NS + k];
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) {
+                                UL[i][v] += pl * c[v];
+                                UR[i][v] += pr * d[v];
+                            }
+                        }
+                    }
+                }
+                const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
+                const double J = a.fgeo[4 * f + 3];
+#pragma unroll
+                for (int i = 0; i < GP; ++i) {
+                    const int g = pg * GP + i;
+                    if (g >= QF) continue;
+                    double out[5] = {0, 0, 0, 0, 0};
+                    if (kind == 1) {
+                        num_flux(a.flux, UL[i], UR[i], n, a.gamma, out);
+                    } else if (kind == 2) {
+                        double Ug[5];
+                        if (a.fbc[f] == FVDG_BC_DIRICHLET) {
+                            const double* ud = a.UD + ((int64_t)a.fdir[f] * QF + g) * 5;
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) Ug[v] = ud[v] + (ud[v] - UL[i][v]);
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) Ug[v] = UL[i][v];
+                            Ug[3] = -UL[i][3];
+                        }
+                        num_flux(a.flux, UL[i], Ug, n, a.gamma, out);
+                    }
+                    const double sc = a.wf[g] * J;
+                    double* o = a.Fh + (f * QF + g) * 5;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) o[v] = out[v] * sc;
+                }
+            }
+        }
+    }
+}
+
+struct FaceTestArgs {
+    SurfArgs s;  // U, R, cgeo, cfaces, fcell, fcombo, parts, accumulate, fused-update fields
+    const double* Fh;
+    const double* PhiC;
+    const uint8_t* canon;
+    const int32_t* perm;
+    int64_t ntiles;
+};
+
+template <int ORDER>
+struct FaceTestShape {
+    static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF;
+    static constexpr int THREADS = 256;
+    static constexpr int TE = THREADS / N > 32 ? 32 : THREADS / N;  // elements per tile
+};
+
+// k_face_test_lds<ORDER,FUSE>: the test pass.  The four local faces are staged one
+// round at a time (TE*Qf*5 doubles of LDS): measured faster than staging all four
+// at once (smaller footprint, more blocks per compute unit).
+template <int ORDER, bool FUSE>
+__global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs A) {
+    using SH = FaceTestShape<ORDER>;
+    constexpr int N = SH::N, QF = SH::QF, NV = 5 * N, TE = SH::TE, NT = SH::THREADS;
+    const SurfArgs& a = A.s;
+    __shared__ double sPhi[8 * QF * N];
+    __shared__ double sF[TE * QF * 5];
+    __shared__ int sPerm[3 * QF];
+    __shared__ int sCanon[24];
+    __shared__ int sFace[TE * 4], sCb[TE * 4];
+
+    const int t = threadIdx.x;
+    for (int i = t; i < 8 * QF * N; i += NT) sPhi[i] = A.PhiC[i];
+    for (int i = t; i < 3 * QF; i += NT) sPerm[i] = A.perm[i];
+    if (t < 24) sCanon[t] = A.canon[t];
+    const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
+
+    for (int64_t tile = blockIdx.x; tile < A.ntiles; tile += gridDim.x) {
+        const int64_t e_base = tile * TE;
+        __syncthreads();
+        if (t < TE * 4) {
+            const int64_t e = e_base + t / 4;
+            int f = -1, cb = 0;
+            if (e < a.E) {
+                const int32_t slot = a.cfaces[e * 4 + (t & 3)];
+                const int ff = slot >> 1, side = slot & 1;
+                const bool bnd = a.fcell[2 * ff + 1] < 0;
+                if ((bnd && do_bnd) || (!bnd && do_int)) f = slot;
+                cb = sCanon[a.fcombo[2 * ff + side]];
+            }
+            sFace[t] = f;
+            sCb[t] = cb;
+        }
+        const int el = t / N, j = t - el * N;
+        const bool active = t < TE * N && e_base + el < a.E;
+        double r[5] = {0, 0, 0, 0, 0};
+#pragma unroll 1
+        for (int lf = 0; lf < 4; ++lf) {
+            __syncthreads();
+            stage_to_lds<TE * QF * 5, NT>(
+                t,
+                [&](int i) {
+                    const int e2 = i / (QF * 5), gv = i - e2 * QF * 5;
+                    const int slot = sFace[e2 * 4 + lf];
+                    if (slot < 0) return 0.0;
+                    const double x = A.Fh[(int64_t)(slot >> 1) * QF * 5 + gv];
+                    return (slot & 1) ? -x : x;
+                },
+                [&](int i, double x) { sF[i] = x; });
+            __syncthreads();
+            if (active && sFace[el * 4 + lf] >= 0) {
+                const int cb = sCb[el * 4 + lf];
+                const double* P = sPhi + (cb / 3) * QF * N + j;
+                const int* pm = sPerm + (cb % 3) * QF;
+                const double* fh = sF + el * QF * 5;
+#pragma unroll 4
+                for (int g = 0; g < QF; ++g) {
+                    const double p = P[pm[g] * N];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) r[v] += p * fh[g * 5 + v];
+                }
+            }
+        }
+        if (active) {
+            const int64_t e = e_base + el;
+            const int64_t i0 = e * NV + 5 * j;
+            if (FUSE) {
+                const double M = a.cgeo[e * 10 + 9] * a.mhat[j];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    const double Rt = a.R[i0 + v] + r[v];
+                    const double sv = a.U[i0 + v] + a.dt * (-Rt / M);
+                    a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) a.R[i0 + v] = a.accumulate ? a.R[i0 + v] + r[v] : r[v];
             }
         }
     }

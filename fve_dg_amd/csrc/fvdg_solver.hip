@@ -8,10 +8,12 @@
 // the neighbour gathers of the face kernel, and the matrix-core volume kernel
 // stages its 16-element tiles through LDS with full-width coalesced loads.
 //
-// Kernels (fvdg_kernels_fast.hpp): k_vol_mfma (volume, FP64 MFMA) and
-// k_surf_tile (faces, owner-computes; fused with the SSP-RK3 stage update inside
-// the time stepper).  Setting FVDG_KERNELS=simple in the environment selects the
-// plain per-point kernels of fvdg_kernels.hpp instead (the GPU cross-check).
+// Kernels (fvdg_kernels_fast.hpp): k_vol_mfma (volume, FP64 MFMA), then the faces in
+// two passes: k_face_flux_lds (every face once: traces + Riemann flux) and
+// k_face_test_lds (test functions per element, fused with the SSP-RK3 stage update
+// inside the time stepper).  Environment switches: FVDG_FACES=owner selects the
+// one-pass owner-computes face kernel k_surf_lds; FVDG_KERNELS=simple selects the
+// plain per-point kernels of fvdg_kernels.hpp (the GPU cross-check).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -93,8 +95,9 @@ struct fvdg_ctx {
     int n_cu = 256;
     // device state: U (state), Ua/Ub (stage ping-pong), R (residual)
     DevBuf<double> U, Ua, Ub, R;
-    DevBuf<double> Fq, Fh;           // scratch of the simple kernels: E*Q*15, F*Qf*5
+    DevBuf<double> Fq, Fh;           // Fq: simple-kernel scratch E*Q*15; Fh: face fluxes F*Qf*5
     bool simple = false;             // FVDG_KERNELS=simple
+    bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
     DevBuf<double> red;              // reduction partials
     int64_t n_red = 0;
 
@@ -165,10 +168,8 @@ int alloc_state(fvdg_ctx* c) {
     HIP_TRY(c->R.alloc(ndof));
     for (DevBuf<double>* b : {&c->U, &c->Ua, &c->Ub, &c->R})
         HIP_TRY(hipMemsetAsync(b->p, 0, ndof * sizeof(double), c->stream));
-    if (c->simple) {
-        HIP_TRY(c->Fq.alloc((size_t)c->E * c->tab.Q * 15));
-        HIP_TRY(c->Fh.alloc((size_t)c->F * c->tab.Qf * 5));
-    }
+    if (c->simple) HIP_TRY(c->Fq.alloc((size_t)c->E * c->tab.Q * 15));
+    HIP_TRY(c->Fh.alloc((size_t)c->F * c->tab.Qf * 5));
     HIP_TRY(c->UD.alloc((size_t)std::max<int64_t>(c->n_dir, 1) * c->tab.Qf * 5));
     HIP_TRY(hipMemsetAsync(c->UD.p, 0, c->UD.n * sizeof(double), c->stream));
     c->n_red = std::max<int64_t>(nblocks(std::max(c->E, c->F * c->tab.Qf), 256), 1);
@@ -237,6 +238,31 @@ SurfArgs surf_args(fvdg_ctx* c, const double* U, double* R, int parts) {
     return a;
 }
 
+// Stream-ordered event pair for per-kernel timing (no-op when ms == nullptr).
+struct Span {
+    hipEvent_t a = nullptr, b = nullptr;
+    double* acc;
+    hipStream_t s;
+    Span(double* ms, hipStream_t st) : acc(ms), s(st) {
+        if (acc) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, s);
+        }
+    }
+    void stop() {
+        if (!acc) return;
+        (void)hipEventRecord(b, s);
+        (void)hipEventSynchronize(b);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, a, b);
+        *acc += t;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        acc = nullptr;
+    }
+};
+
 template <int P>
 void launch_vol_fast(fvdg_ctx* c, const double* U, double* R) {
     VolArgs v{c->E, c->gamma, U, R, c->cgeo.p, c->tVA.p, c->tGA.p, c->tWqa.p};
@@ -262,7 +288,40 @@ void dispatch_vol(fvdg_ctx* c, const double* U, double* R) {
         case 4: launch_vol_fast<4>(c, U, R); break;
     }
 }
-void dispatch_surf(fvdg_ctx* c, const SurfArgs& a, bool fuse) {
+template <int P>
+void launch_faces_two_pass(fvdg_ctx* c, const SurfArgs& a, bool fuse, double* ms_flux, double* ms_test) {
+    FaceFluxArgs f{c->F, c->gamma, c->flux, a.U, c->fcell.p, c->fcombo.p, c->fgeo.p, c->fbc.p, c->fdir.p, c->UD.p,
+                   c->tWf.p, c->tPhiC.p, c->tCanon.p, c->tPerm.p, c->Fh.p, 0};
+    constexpr int FB = FaceFluxShape<P>::FB;
+    f.ntiles = (c->F + FB - 1) / FB;
+    const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * 8);
+    Span s1(ms_flux, c->stream);
+    hipLaunchKernelGGL(k_face_flux_lds<P>, dim3(g1), dim3(FaceFluxShape<P>::THREADS), 0, c->stream, f);
+    s1.stop();
+    Span s2(ms_test, c->stream);
+    constexpr int TE = FaceTestShape<P>::TE;
+    FaceTestArgs t{a, c->Fh.p, c->tPhiC.p, c->tCanon.p, c->tPerm.p, (c->E + TE - 1) / TE};
+    const unsigned g2 = (unsigned)std::min<int64_t>(t.ntiles, (int64_t)c->n_cu * 8);
+    if (fuse)
+        hipLaunchKernelGGL((k_face_test_lds<P, true>), dim3(g2), dim3(FaceTestShape<P>::THREADS), 0, c->stream, t);
+    else
+        hipLaunchKernelGGL((k_face_test_lds<P, false>), dim3(g2), dim3(FaceTestShape<P>::THREADS), 0, c->stream, t);
+    s2.stop();
+}
+
+// ms_flux / ms_test (nullable): event-timed face-flux and face-test(+update) kernels;
+// the one-pass owner-computes kernel is accounted under ms_flux.
+void dispatch_surf(fvdg_ctx* c, const SurfArgs& a, bool fuse, double* ms_flux = nullptr, double* ms_test = nullptr) {
+    if (!c->owner_faces) {
+        switch (c->order) {
+            case 0: launch_faces_two_pass<0>(c, a, fuse, ms_flux, ms_test); return;
+            case 1: launch_faces_two_pass<1>(c, a, fuse, ms_flux, ms_test); return;
+            case 2: launch_faces_two_pass<2>(c, a, fuse, ms_flux, ms_test); return;
+            case 3: launch_faces_two_pass<3>(c, a, fuse, ms_flux, ms_test); return;
+            case 4: launch_faces_two_pass<4>(c, a, fuse, ms_flux, ms_test); return;
+        }
+    }
+    Span so(ms_flux, c->stream);
     switch (c->order) {
         case 0: launch_surf_fast<0>(c, a, fuse); break;
         case 1: launch_surf_fast<1>(c, a, fuse); break;
@@ -270,32 +329,8 @@ void dispatch_surf(fvdg_ctx* c, const SurfArgs& a, bool fuse) {
         case 3: launch_surf_fast<3>(c, a, fuse); break;
         case 4: launch_surf_fast<4>(c, a, fuse); break;
     }
+    so.stop();
 }
-
-// Stream-ordered event pair for per-kernel timing (no-op when ms == nullptr).
-struct Span {
-    hipEvent_t a = nullptr, b = nullptr;
-    double* acc;
-    hipStream_t s;
-    Span(double* ms, hipStream_t st) : acc(ms), s(st) {
-        if (acc) {
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, s);
-        }
-    }
-    void stop() {
-        if (!acc) return;
-        (void)hipEventRecord(b, s);
-        (void)hipEventSynchronize(b);
-        float t = 0;
-        (void)hipEventElapsedTime(&t, a, b);
-        *acc += t;
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        acc = nullptr;
-    }
-};
 
 // R = selected parts of the residual of U (device buffers)
 int launch_residual(fvdg_ctx* c, const double* U, double* R, int parts, double* ms_vol, double* ms_surf) {
@@ -331,7 +366,7 @@ int launch_residual(fvdg_ctx* c, const double* U, double* R, int parts, double* 
 // L = -M^{-1} R.  Stage inputs and outputs ping-pong between Ua and Ub so that no
 // kernel writes a buffer whose neighbour values other blocks may still read; the
 // last stage writes U in place (each thread reads U0[i] before writing out[i]).
-// ms (nullable): [1] volume, [2] surface + update, accumulated.
+// ms (nullable): [1] volume, [2] face flux, [3] face test + stage update, accumulated.
 int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
     static const double A[3] = {0.0, 0.75, 1.0 / 3.0};
     static const double B[3] = {1.0, 0.25, 2.0 / 3.0};
@@ -341,7 +376,7 @@ int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
         if (c->simple) {
             int rc = launch_residual(c, src[s], c->R.p, FVDG_PART_ALL, ms ? &ms[1] : nullptr, ms ? &ms[2] : nullptr);
             if (rc) return rc;
-            Span su(ms ? &ms[2] : nullptr, c->stream);
+            Span su(ms ? &ms[3] : nullptr, c->stream);
             const int64_t n = c->E * 5 * c->tab.N;
             hipLaunchKernelGGL(k_rk_update, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, c->U.p, src[s], c->R.p,
                                dst[s], c->cgeo.p, c->tMhat.p, c->E, c->tab.N, A[s], B[s], dt);
@@ -350,15 +385,13 @@ int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
             Span sv(ms ? &ms[1] : nullptr, c->stream);
             dispatch_vol(c, src[s], c->R.p);
             sv.stop();
-            Span ss(ms ? &ms[2] : nullptr, c->stream);
             SurfArgs a = surf_args(c, src[s], c->R.p, FVDG_PART_ALL);
             a.U0 = c->U.p;
             a.out = dst[s];
             a.a0 = A[s];
             a.b0 = B[s];
             a.dt = dt;
-            dispatch_surf(c, a, true);
-            ss.stop();
+            dispatch_surf(c, a, true, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
         }
         HIP_TRY(hipGetLastError());
     }
@@ -383,6 +416,8 @@ int fvdg_create(fvdg_ctx** out, int device) {
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     const char* kern = std::getenv("FVDG_KERNELS");
     c->simple = kern && std::strcmp(kern, "simple") == 0;
+    const char* faces = std::getenv("FVDG_FACES");
+    c->owner_faces = faces && std::strcmp(faces, "owner") == 0;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();
     return FVDG_OK;
@@ -668,7 +703,6 @@ int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms) {
         if (rc) return rc;
     }
     for (int i = 1; i < 4; ++i) ms[i] = split[i] / n_stages;
-    // the tuned path fuses the stage update into the face kernel: ms[3] stays 0
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return FVDG_OK;

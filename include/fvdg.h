@@ -139,8 +139,10 @@ int fvdg_ssprk3_step(fvdg_ctx* c, double dt, int n_steps);
 int fvdg_max_wave_speed(fvdg_ctx* c, double* out);
 /* sum over elements of |det A_e| * U_{e,k=0,v} / 6 = integral of each conserved variable */
 int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);
-/* Time n_stages SSP-RK3 stages (launches only, no host sync inside) with HIP events on the
- * solver stream; ms[0] = total, ms[1..3] = volume, surface, update kernels (averaged per stage). */
+/* Time n_stages SSP-RK3 stages with HIP events on the solver stream: ms[0] = wall time per
+ * stage of a pass with no host synchronisation inside; then a second pass with events between
+ * kernels gives ms[1] = volume, ms[2] = face flux, ms[3] = face test + stage update
+ * (per-stage averages). */
 int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms);
 
 #ifdef __cplusplus
