@@ -34,22 +34,29 @@ WORKLOADS = {
 }
 VORTEX = [1.4, 5.0, 5.0, 5.0, 1.0, 1.0, 10.0, 10.0]
 
-# HBM bytes per launch at the default workload from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-# (factors from a 1 GiB device copy); see profiles/.  Filled in per kernel when measured.
-TRAFFIC = {}
+# HBM bytes per launch at the default workload (vortex_p3_196k) from rocprofv3 FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), profiles/dg_stage_p3_196k_fp64.md
+TRAFFIC = {
+    "vortex_p3_196k": {
+        "volume (k_vol_mfma)": (85037 * 2 + 153600) * 1024,
+        "face flux (k_face_flux_lds)": (167752 * 2 + 368656) * 1024,
+        "face test + update (k_face_test_lds)": (476464 * 2 + 164627) * 1024,
+    }
+}
 
 
 def algorithmic(order, Q, Qf, N):
-    """Algorithmic work per element and stage (FMA = 2 flops):
-    volume kernel      40QN + 145Q   (interpolation 10QN, test 30QN, pointwise flux ~145Q)
-    face-flux kernel   40QfN + 410Qf (2 faces x [2 traces 10QfN each + Riemann ~205Qf])
-    face-test kernel   40QfN + 25N   (4 face sides x 10QfN test sums + the stage update)
-    bytes of one stage 8*5N*(2 + 2/3) + 168 (state in/out, U^n in 2 of 3 stages, geometry)."""
+    """Algorithmic work per element and stage (FMA = 2 flops, 8-byte words).
+    volume kernel      40QN + 145Q flops (interpolation 10QN, test 30QN, pointwise flux ~145Q);
+                       intensity ~17 flop/B at P3: FP64-compute bound
+    face-flux kernel   40QfN + 410Qf flops (2 faces x [2 traces 10QfN + Riemann ~205Qf]);
+                       ~11 flop/B: at the FP64 ridge, counted against FP64
+    face-test kernel   reads U, R, 4 face-flux blocks, U^n (2 of 3 stages), |det|; writes U:
+                       8*(5N*(3 + 2/3) + 4*Qf*5 + 1) bytes, ~3 flop/B: HBM bound."""
     vol_flops = 40 * Q * N + 145 * Q
     flux_flops = 40 * Qf * N + 410 * Qf
-    test_flops = 40 * Qf * N + 25 * N
-    stage_bytes = 8 * 5 * N * (2 + 2 / 3) + 168
-    return vol_flops, flux_flops, test_flops, stage_bytes
+    test_bytes = 8 * (5 * N * (3 + 2 / 3) + 4 * Qf * 5 + 1)
+    return vol_flops, flux_flops, test_bytes
 
 
 def cpu_baseline(workload_n, order, flux, seconds_target=15.0):
@@ -134,7 +141,7 @@ def main():
     dof = 5 * N * E
     value = dof / (stage_ms * 1e-3)
 
-    vol_f, flux_f, test_f, stage_b = algorithmic(order, Q, Qf, N)
+    vol_f, flux_f, test_b = algorithmic(order, Q, Qf, N)
 
     def rate(work, t_ms):
         return work * E / (t_ms * 1e-3) if t_ms > 0 else 0.0
@@ -144,14 +151,15 @@ def main():
                                 "unit": "TFLOP/s", "scale": 1e12},
         "face flux (k_face_flux_lds)": {"ms": ms[2], "achieved": rate(flux_f, ms[2]), "peak": PEAK_FP64,
                                         "bound": "mfma", "unit": "TFLOP/s", "scale": 1e12},
-        "face test + update (k_face_test_lds)": {"ms": ms[3], "achieved": rate(test_f, ms[3]), "peak": PEAK_FP64,
-                                                 "bound": "mfma", "unit": "TFLOP/s", "scale": 1e12},
+        "face test + update (k_face_test_lds)": {"ms": ms[3], "achieved": rate(test_b, ms[3]), "peak": PEAK_HBM,
+                                                 "bound": "hbm", "unit": "GB/s", "scale": 1e9},
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     k = kernels[dom]
     roofline = {"bound": k["bound"], "achieved": round(k["achieved"] / k["scale"], 3),
                 "peak": round(k["peak"] / k["scale"], 1), "unit": k["unit"],
-                "frac": round(k["achieved"] / k["peak"], 4), "traffic": TRAFFIC.get(dom),
+                "frac": round(k["achieved"] / k["peak"], 4), "traffic": TRAFFIC.get(args.workload, {}).get(dom),
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
                 "kernel": dom, "kernel_ms": round(k["ms"], 4),
                 "all_kernels": {name: {"ms": round(v["ms"], 4), "achieved": round(v["achieved"] / v["scale"], 3),
                                        "unit": v["unit"], "frac": round(v["achieved"] / v["peak"], 4)}

@@ -21,6 +21,10 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 FIELD = {"vortex": 0, "tgv": 1, "const": 2}
 PART_CELLS, PART_INTERNALS, PART_BOUNDARYS, PART_ALL = 1, 2, 4, 7
 BC_DIRICHLET, BC_PSEUDO3DZ = 0, 5
+FVDG_BC_DIRICHLET, FVDG_BC_PSEUDO3DZ = BC_DIRICHLET, BC_PSEUDO3DZ
+# status codes of include/fvdg.h
+FVDG_OK, FVDG_ERR_ARG, FVDG_ERR_HIP, FVDG_ERR_MESH = 0, -1, -2, -3
+FVDG_ERR_STATE, FVDG_ERR_UNSUPPORTED, FVDG_ERR_NONFINITE = -4, -5, -6
 
 _d = C.POINTER(C.c_double)
 _i64 = C.POINTER(C.c_int64)
@@ -81,6 +85,17 @@ def lib():
         _sig(L, "fvdg_time_stages", [_vp, C.c_double, C.c_int, _d])
         _lib = L
     return _lib
+
+
+def gpu_available():
+    """True when the HIP runtime sees a device (asks torch, which counts devices without
+    initialising one)."""
+    try:
+        import torch
+
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
 
 
 def _check(status):

@@ -1,5 +1,5 @@
 """Build recipe for libfvdg.so (hipcc, gfx950) -- run from the repository root
-or through ``__graft_entry__.build()``.
+or through ``__graft_entry__.build()`` (which also builds the test checker under oracle/).
 
     python -m fve_dg_amd.build
 """
@@ -41,15 +41,6 @@ def build_library(verbose=False):
     return LIB
 
 
-def build_oracle(verbose=False):
-    res = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "-j4"], capture_output=True, text=True)
-    if verbose:
-        print(res.stdout, res.stderr)
-    if res.returncode != 0:
-        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
-
-
 if __name__ == "__main__":
     build_library(verbose=True)
-    build_oracle(verbose=True)
     print("built", LIB, file=sys.stderr)

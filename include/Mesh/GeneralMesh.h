@@ -137,14 +137,18 @@ public:
     // stay boundary faces.  Returns the number of merged pairs.
     uInt make_periodic(const vector3f& lb, const vector3f& ub, std::array<bool, 3> axes) {
         const std::vector<uInt> cls = periodic_classes(lb, ub, axes);
-        std::unordered_map<Key3, uInt, Key3Hash> first;
+        const Scalar tol = plane_tolerance(lb, ub), q = min_edge() / 64;
+        // Faces are matched by their wrapped centroid: on a grid with two cells along a
+        // periodic axis, distinct faces of one plane can share all three node classes.
+        std::map<std::array<int64_t, 3>, uInt> first;
         std::vector<uInt> remap(m_faces.size(), uInt(-1));
         std::vector<char> drop(m_faces.size(), 0);
         uInt merged = 0;
         for (uInt f = 0; f < m_faces.size(); ++f) {
             auto* tf = std::get_if<TriangleFace>(&m_faces[f]);
             if (!tf || tf->neighbor_cells[1] != uInt(-1)) continue;
-            const Key3 key = sorted_key({cls[tf->nodes[0]], cls[tf->nodes[1]], cls[tf->nodes[2]]});
+            vector3f c = (m_points[tf->nodes[0]] + m_points[tf->nodes[1]] + m_points[tf->nodes[2]]) / Scalar(3);
+            const std::array<int64_t, 3> key = wrapped_key(c, lb, ub, axes, tol, q);
             auto it = first.find(key);
             if (it == first.end()) {
                 first.emplace(key, f);
@@ -235,16 +239,20 @@ private:
         const Scalar q = min_edge() / 64;
         std::map<std::array<int64_t, 3>, uInt> cls_of;
         std::vector<uInt> cls(m_points.size());
-        for (uInt p = 0; p < m_points.size(); ++p) {
-            std::array<int64_t, 3> key;
-            for (int d = 0; d < 3; ++d) {
-                Scalar x = m_points[p][d];
-                if (axes[d] && std::abs(x - ub[d]) <= tol) x = lb[d];
-                key[d] = (int64_t)std::llround((x - lb[d]) / q);
-            }
-            cls[p] = cls_of.emplace(key, (uInt)cls_of.size()).first->second;
-        }
+        for (uInt p = 0; p < m_points.size(); ++p)
+            cls[p] = cls_of.emplace(wrapped_key(m_points[p], lb, ub, axes, tol, q), (uInt)cls_of.size()).first->second;
         return cls;
+    }
+    // A position wrapped onto [lb, ub) along the periodic axes, quantised to q.
+    static std::array<int64_t, 3> wrapped_key(const vector3f& p, const vector3f& lb, const vector3f& ub,
+                                              std::array<bool, 3> axes, Scalar tol, Scalar q) {
+        std::array<int64_t, 3> key;
+        for (int d = 0; d < 3; ++d) {
+            Scalar x = p[d];
+            if (axes[d] && std::abs(x - ub[d]) <= tol) x = lb[d];
+            key[d] = (int64_t)std::llround((x - lb[d]) / q);
+        }
+        return key;
     }
 
     using Key3 = std::array<uInt, 3>;

@@ -70,6 +70,7 @@ typedef struct fvdg_ctx fvdg_ctx;   /* one GPU, one stream, one mesh, one scheme
 
 const char* fvdg_version(void);
 const char* fvdg_last_error(void);
+/* (P+1)(P+2)(P+3)/6 modes per variable (DGBasisEvaluator<P>::NumBasis); 0 for order < 0 */
 int fvdg_num_basis(int order);
 /* values val[N] and reference gradients grad[3N] (either may be NULL) of the order-`order`
  * modal basis at reference point (x,y,z) (DGBasisEvaluator<P>::eval_all / grad_all,

@@ -167,3 +167,30 @@ def test_tuned_kernels_match_simple_kernels_full_size(solver, order, flux, n):
         assert rel_err(solver.download(), simple.download()) < 1e-13
     finally:
         simple.close()
+
+
+@pytest.mark.parametrize("mesh_name", ["ref_box6", "ref_box5"])
+@pytest.mark.parametrize("order,flux", [(1, "lf"), (3, "roe"), (4, "lf")])
+def test_reference_meshes_divergence_identity(solver, oracle, mesh_name, order, flux):
+    """On the meshes the reference's own mesh code builds (tests/golden), with a polynomial
+    state and exact Dirichlet data: GPU = oracle to 1e-12, and GPU = int (div F) phi_j."""
+    from test_oracle_residual import MESHES, LinearState, divergence_rhs, face_scale, project
+
+    print(f"[case] {mesh_name} order={order} flux={flux}", flush=True)
+    m = MESHES[mesh_name]
+    mesh = F.Mesh.from_arrays(m["points"], m["cells"], m["face_nodes"], m["face_cells"])
+    st = LinearState(seed=20 + order)
+    U = project(m, order, st.state)
+    solver.set_scheme(order, flux=flux).set_mesh(mesh)
+    solver.upload(U)
+    _, xyz = solver.dirichlet_points()
+    UD = st.state(xyz.reshape(-1, 3)).reshape(xyz.shape[:2] + (5,))
+    solver.set_dirichlet_states(UD)
+    Rg = solver.residual()
+    bc = mesh.boundary_types()
+    Ro = oracle.residual(m, order, U, flux=flux, bc=bc, UD=UD)
+    assert rel_err(Rg, Ro) < TOL
+    og = oracle.geometry(m)
+    Rx = divergence_rhs(m, order, og["det"], st.div_flux)
+    scale = np.abs(Rx).max() if order == 1 else face_scale(oracle, m, og, U)
+    assert np.abs(Rg - Rx).max() <= (1e-12 if order == 1 else 1e-9) * scale
