@@ -17,7 +17,7 @@ namespace fvdg {
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 
-constexpr int kMaxOrder = 4;  // device kernels are instantiated for orders 0..4
+constexpr int kMaxOrder = 5;  // device kernels are instantiated for orders 0..5
 
 // The 24 ordered triples (a0,a1,a2) of distinct reference-tet vertices, in
 // lexicographic order.  A face seen from one of its cells is one such triple:

@@ -119,9 +119,16 @@ __device__ __forceinline__ void flux_lf(const double UL[5], const double UR[5], 
 
 // Roe (1981) flux with Harten's entropy fix on the acoustic waves, delta = 0.1 * a_roe:
 //   Fhat = 0.5 (F_L n + F_R n) - 0.5 sum_k |lambda_k| alpha_k r_k
+// The Roe average of two physical states (rho > 0, p > 0) has a positive sound speed;
+// for a non-physical trace (an overshooting high-order polynomial) it may not, and the
+// wave strengths dp / a^2 blow up, so such points take the Lax-Friedrichs flux instead.
 __device__ __forceinline__ void flux_roe(const double UL[5], const double UR[5], const double n[3], double gamma,
                                          double out[5]) {
     const Prim L = euler_prim(UL, gamma), R = euler_prim(UR, gamma);
+    if (!(UL[0] > 0.0 && UR[0] > 0.0 && L.p > 0.0 && R.p > 0.0)) {
+        flux_lf(UL, UR, n, gamma, out);
+        return;
+    }
     double fl[5], fr[5];
     euler_flux_n(L, n, fl);
     euler_flux_n(R, n, fr);

@@ -54,6 +54,7 @@ template <> struct Ord<1> { static constexpr int N = 4, Q = 15, QF = 7; };
 template <> struct Ord<2> { static constexpr int N = 10, Q = 15, QF = 24; };
 template <> struct Ord<3> { static constexpr int N = 20, Q = 31, QF = 24; };
 template <> struct Ord<4> { static constexpr int N = 35, Q = 57, QF = 24; };
+template <> struct Ord<5> { static constexpr int N = 56, Q = 94, QF = 36; };
 
 template <int ORDER>
 struct VolShape {
@@ -228,7 +229,7 @@ struct SurfLdsArgs {
 template <int ORDER>
 struct SurfLdsShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF;
-    static constexpr int TE = N > 20 ? 8 : 16;  // elements per tile
+    static constexpr int TE = N > 35 ? 4 : (N > 20 ? 8 : 16);  // elements per tile
     static constexpr int GP = 2;                // face points per lane in the flux phase
     static constexpr int NPG = (QF + GP - 1) / GP;
     static constexpr int THREADS = 256;
@@ -455,7 +456,8 @@ template <int ORDER>
 struct FaceFluxShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF;
     static constexpr int GP = 2, LPF = (QF + GP - 1) / GP;  // lanes per face
-    static constexpr int THREADS = 192;
+    // P5's table (8 x 36 x 57 doubles, 131 KB) leaves room for only a small tile
+    static constexpr int THREADS = N > 35 ? 64 : 192;
     static constexpr int FB = THREADS / LPF;                // faces per tile
 };
 

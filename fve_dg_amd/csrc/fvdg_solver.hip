@@ -77,6 +77,7 @@ struct fvdg_ctx {
     std::vector<int64_t> dir_faces;       // Dirichlet face ids, host copy
     std::vector<double> dir_xyz;          // their face quadrature points
     std::vector<double> cell_det_host;    // for the mass diagonal
+    double h_min = 0;                     // min over cells of 3 * volume / (sum of face areas)
     // device geometry
     DevBuf<double> cgeo;        // E*10: A^{-1} (9), |det A|
     DevBuf<int32_t> cfaces;     // E*4: face*2 + side
@@ -286,6 +287,7 @@ void dispatch_vol(fvdg_ctx* c, const double* U, double* R) {
         case 2: launch_vol_fast<2>(c, U, R); break;
         case 3: launch_vol_fast<3>(c, U, R); break;
         case 4: launch_vol_fast<4>(c, U, R); break;
+        case 5: launch_vol_fast<5>(c, U, R); break;
     }
 }
 template <int P>
@@ -319,6 +321,7 @@ void dispatch_surf(fvdg_ctx* c, const SurfArgs& a, bool fuse, double* ms_flux = 
             case 2: launch_faces_two_pass<2>(c, a, fuse, ms_flux, ms_test); return;
             case 3: launch_faces_two_pass<3>(c, a, fuse, ms_flux, ms_test); return;
             case 4: launch_faces_two_pass<4>(c, a, fuse, ms_flux, ms_test); return;
+            case 5: launch_faces_two_pass<5>(c, a, fuse, ms_flux, ms_test); return;
         }
     }
     Span so(ms_flux, c->stream);
@@ -328,6 +331,7 @@ void dispatch_surf(fvdg_ctx* c, const SurfArgs& a, bool fuse, double* ms_flux = 
         case 2: launch_surf_fast<2>(c, a, fuse); break;
         case 3: launch_surf_fast<3>(c, a, fuse); break;
         case 4: launch_surf_fast<4>(c, a, fuse); break;
+        case 5: launch_surf_fast<5>(c, a, fuse); break;
     }
     so.stop();
 }
@@ -367,12 +371,12 @@ int launch_residual(fvdg_ctx* c, const double* U, double* R, int parts, double* 
 // kernel writes a buffer whose neighbour values other blocks may still read; the
 // last stage writes U in place (each thread reads U0[i] before writing out[i]).
 // ms (nullable): [1] volume, [2] face flux, [3] face test + stage update, accumulated.
-int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
+int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
     static const double A[3] = {0.0, 0.75, 1.0 / 3.0};
     static const double B[3] = {1.0, 0.25, 2.0 / 3.0};
     const double* src[3] = {c->U.p, c->Ua.p, c->Ub.p};
     double* dst[3] = {c->Ua.p, c->Ub.p, c->U.p};
-    for (int s = 0; s < 3; ++s) {
+    {
         if (c->simple) {
             int rc = launch_residual(c, src[s], c->R.p, FVDG_PART_ALL, ms ? &ms[1] : nullptr, ms ? &ms[2] : nullptr);
             if (rc) return rc;
@@ -394,6 +398,14 @@ int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
             dispatch_surf(c, a, true, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
         }
         HIP_TRY(hipGetLastError());
+    }
+    return FVDG_OK;
+}
+
+int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
+    for (int s = 0; s < 3; ++s) {
+        const int rc = ssprk3_stage(c, dt, s, ms);
+        if (rc) return rc;
     }
     return FVDG_OK;
 }
@@ -432,7 +444,7 @@ void fvdg_destroy(fvdg_ctx* c) {
 
 int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precision) {
     if (!c) return fail(FVDG_ERR_ARG, "null context");
-    if (order < 0 || order > kMaxOrder) return fail(FVDG_ERR_UNSUPPORTED, "order must be 0..4");
+    if (order < 0 || order > kMaxOrder) return fail(FVDG_ERR_UNSUPPORTED, "order must be 0..5");
     if (flux != FVDG_FLUX_LF && flux != FVDG_FLUX_ROE) return fail(FVDG_ERR_ARG, "unknown flux");
     if (!(gamma > 1.0)) return fail(FVDG_ERR_ARG, "gamma must exceed 1");
     if (precision != FVDG_PREC_FP64) return fail(FVDG_ERR_UNSUPPORTED, "only FVDG_PREC_FP64 is available");
@@ -462,8 +474,10 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
     std::vector<double> cgeo(E * 10);
     std::vector<int32_t> cfaces(E * 4);
     std::vector<double> det(E);
+    double h_min = INFINITY;
     for (int64_t e = 0; e < E; ++e) {
         const auto& t = cm.m_cells[e];
+        h_min = std::min(h_min, 3.0 * t.m_h);
         for (int i = 0; i < 9; ++i) cgeo[e * 10 + i] = t.m_invJac[i];
         cgeo[e * 10 + 9] = t.compute_jacobian_det();
         det[e] = cgeo[e * 10 + 9];
@@ -514,6 +528,7 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
     c->dir_faces = dir_faces;
     c->dir_xyz.clear();
     c->cell_det_host = det;
+    c->h_min = h_min;
     HIP_TRY(c->cgeo.upload(cgeo, c->stream));
     HIP_TRY(c->cfaces.upload(cfaces, c->stream));
     HIP_TRY(c->fcell.upload(fcell, c->stream));
@@ -626,6 +641,27 @@ int fvdg_ssprk3_step(fvdg_ctx* c, double dt, int n_steps) {
         if (rc) return rc;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
+
+int fvdg_ssprk3_stage(fvdg_ctx* c, double dt, int stage) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (stage < 0 || stage > 2 || !(dt >= 0)) return fail(FVDG_ERR_ARG, "bad dt or stage index");
+    HIP_TRY(hipSetDevice(c->device));
+    rc = ssprk3_stage(c, dt, stage, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
+
+int fvdg_stable_dt(fvdg_ctx* c, double cfl, double* dt) {
+    if (!dt || !(cfl > 0)) return fail(FVDG_ERR_ARG, "bad cfl or null output");
+    double lam = 0;
+    int rc = fvdg_max_wave_speed(c, &lam);
+    if (rc) return rc;
+    if (!(lam > 0) || !std::isfinite(lam)) return fail(FVDG_ERR_NONFINITE, "maximum wave speed is not positive and finite");
+    *dt = cfl * c->h_min / ((2 * c->order + 1) * lam);
     return FVDG_OK;
 }
 

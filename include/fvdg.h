@@ -118,7 +118,7 @@ int fvdg_analytic_error(const fvdg_mesh* m, int order, int field, const double* 
 /* ---------------------------- device solver ---------------------------- */
 int fvdg_create(fvdg_ctx** out, int device);
 void fvdg_destroy(fvdg_ctx* c);
-/* order 0..4; flux FVDG_FLUX_*; gamma > 1; precision FVDG_PREC_* */
+/* order 0..5; flux FVDG_FLUX_*; gamma > 1; precision FVDG_PREC_* */
 int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precision);
 /* Copies the mesh geometry to the device; the mesh object may be destroyed afterwards. */
 int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m);
@@ -136,6 +136,14 @@ int fvdg_residual(fvdg_ctx* c, double t, int parts, double* R);
 int fvdg_mass_diagonal(fvdg_ctx* c, double* M);
 /* n_steps SSP-RK3 steps of dU/dt = -M^{-1} R(U) on the device state. */
 int fvdg_ssprk3_step(fvdg_ctx* c, double dt, int n_steps);
+/* stage 0, 1 or 2 of one Shu-Osher SSP-RK3 step on the device state (U^n -> U1 -> U2 -> U^{n+1});
+ * calling 0, 1, 2 in order equals one fvdg_ssprk3_step.  Between stages the caller may
+ * update the Dirichlet states to the stage times t, t + dt, t + dt/2. */
+int fvdg_ssprk3_stage(fvdg_ctx* c, double dt, int stage);
+/* dt = cfl * h_min / ((2p+1) * lambda_max), h = 3 * volume / (sum of the cell's face areas),
+ * lambda_max = max over face points of max_{L,R}(|u| + c) of the device state
+ * (examples/Sedov/SedovBatch.cpp:174-199 form, with the tet inradius as h) */
+int fvdg_stable_dt(fvdg_ctx* c, double cfl, double* dt);
 /* max over all face quadrature points of max(|u|+c) of the two traces */
 int fvdg_max_wave_speed(fvdg_ctx* c, double* out);
 /* sum over elements of |det A_e| * U_{e,k=0,v} / 6 = integral of each conserved variable */

@@ -169,11 +169,17 @@ void or_lf(const double UL[5], const double UR[5], const double n[3], double gam
 }
 /* Roe flux 0.5(F_L n + F_R n) - 0.5 sum_k |l_k| alpha_k r_k with Harten's fix (delta = 0.1 a)
  * on the acoustic waves.  signed_eigs != 0 returns instead sum_k l_k alpha_k r_k (no abs, no
- * fix), which equals (F_R - F_L) n exactly for the Roe average (tests pin the decomposition). */
+ * fix), which equals (F_R - F_L) n exactly for the Roe average (tests pin the decomposition).
+ * A non-physical state (rho <= 0 or p <= 0) takes the Lax-Friedrichs flux: the Roe average
+ * is only guaranteed a positive sound speed for physical states. */
 void or_roe(const double UL[5], const double UR[5], const double n[3], double gamma, double out[5],
             int signed_eigs) {
     double rL = positive(UL[0]), rR = positive(UR[0]);
     double pL = or_pressure(UL, gamma), pR = or_pressure(UR, gamma);
+    if (!signed_eigs && !(UL[0] > 0 && UR[0] > 0 && pL > 0 && pR > 0)) {
+        or_lf(UL, UR, n, gamma, out);
+        return;
+    }
     double uL[3] = {UL[1] / rL, UL[2] / rL, UL[3] / rL}, uR[3] = {UR[1] / rR, UR[2] / rR, UR[3] / rR};
     double HL = (UL[4] + pL) / rL, HR = (UR[4] + pR) / rR;
     double sL = sqrt(rL), sR = sqrt(rR);

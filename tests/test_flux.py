@@ -136,3 +136,20 @@ def test_inverse3_matches_reference(oracle, ref):
         a, b = oracle.inverse3(A), ref.inverse3(A)
         assert np.abs(a - b).max() <= 1e-15 * np.abs(b).max()
         assert np.abs(oracle.inverse3(A) @ A.reshape(3, 3) - np.eye(3)).max() < 1e-12
+
+
+@pytest.mark.parametrize("bad", ["negative_energy", "negative_density"])
+def test_roe_non_physical_state_takes_lax_friedrichs(oracle, bad):
+    """A trace with p <= 0 or rho <= 0 has no real Roe-averaged sound speed; such points
+    use the Lax-Friedrichs flux (the state below is a P3 face trace met on a coarse mesh)."""
+    UL = np.array([0.5412775815615278, 0.22413491212667186, 1.0954546706020096, 0.0, 1.4438236016248314])
+    UR = np.array([0.27407240678777567, -0.4779972781645934, 1.167592766300288, 0.0, -0.07210789244984943])
+    if bad == "negative_density":
+        UR = np.array([-0.01, 0.0, 0.1, 0.0, 1.0])
+    n = np.array([0.0, -0.4472135954999579, 0.8944271909999159])
+    roe = oracle.numflux("roe", UL, UR, n)
+    assert np.isfinite(roe).all()
+    assert np.array_equal(roe, oracle.numflux("lf", UL, UR, n))
+    # physical states keep the Roe flux
+    UR2 = np.array([0.6, 0.1, 1.0, 0.0, 2.0])
+    assert not np.allclose(oracle.numflux("roe", UL, UR2, n), oracle.numflux("lf", UL, UR2, n))

@@ -32,7 +32,7 @@ def dirichlet_states(solver, oracle, params, t=0.0):
     return UD
 
 
-@pytest.mark.parametrize("order", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("flux", ["lf", "roe"])
 def test_residual_periodic_jittered(solver, oracle, order, flux):
     print(f"[case] periodic jittered n=3 order={order} flux={flux}", flush=True)
@@ -87,7 +87,7 @@ def test_state_roundtrip_and_mass(solver, oracle):
     assert rel_err(solver.mass(), oracle.mass(mesh.export(), 2)) < 1e-14
 
 
-@pytest.mark.parametrize("order,flux", [(1, "lf"), (3, "roe"), (4, "lf")])
+@pytest.mark.parametrize("order,flux", [(1, "lf"), (3, "roe"), (4, "lf"), (5, "roe")])
 def test_ssprk3_matches_oracle(solver, oracle, order, flux):
     print(f"[case] ssprk3 order={order} flux={flux}", flush=True)
     mesh = F.Mesh.box(3, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=7)
