@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="vortex_p3_196k", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="contraction precision: fp64 (the drop-in) or fp32 (mixed: FP32 volume/face "
+                         "contractions, FP64 fluxes, update and state)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before the run, copy MIB MiB device-to-device once (a plain copy kernel of known "
@@ -130,7 +133,7 @@ def main():
     vp, vw, fp, fw = F.quadrature(order)
     Q, Qf = len(vw), len(fw)
     s = F.Solver(0)
-    s.set_scheme(order, flux=flux).set_mesh(mesh)
+    s.set_scheme(order, flux=flux, precision=args.precision).set_mesh(mesh)
     s.upload(U)
     h = mesh.geometry()["h"].min() * 3.0
     dt = 0.3 * h / ((2 * order + 1) * s.max_wave_speed())

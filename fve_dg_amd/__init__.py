@@ -80,6 +80,8 @@ def lib():
         _sig(L, "fvdg_residual", [_vp, C.c_double, C.c_int, _d])
         _sig(L, "fvdg_mass_diagonal", [_vp, _d])
         _sig(L, "fvdg_ssprk3_step", [_vp, C.c_double, C.c_int])
+        _sig(L, "fvdg_ssprk3_stage", [_vp, C.c_double, C.c_int])
+        _sig(L, "fvdg_stable_dt", [_vp, C.c_double, _d])
         _sig(L, "fvdg_max_wave_speed", [_vp, _d])
         _sig(L, "fvdg_conservation_sums", [_vp, _d])
         _sig(L, "fvdg_time_stages", [_vp, C.c_double, C.c_int, _d])
@@ -285,6 +287,16 @@ class Solver:
 
     def ssprk3(self, dt, n_steps=1):
         _check(lib().fvdg_ssprk3_step(self._h, dt, n_steps))
+
+    def ssprk3_stage(self, dt, stage):
+        """stage 0, 1 or 2 of one SSP-RK3 step (Dirichlet states may be reset in between)"""
+        _check(lib().fvdg_ssprk3_stage(self._h, dt, stage))
+
+    def stable_dt(self, cfl=0.3):
+        """cfl * h_min / ((2p+1) lambda_max) of the device state"""
+        out = C.c_double()
+        _check(lib().fvdg_stable_dt(self._h, cfl, C.byref(out)))
+        return out.value
 
     def max_wave_speed(self):
         out = C.c_double()

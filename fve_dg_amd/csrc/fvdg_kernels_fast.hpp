@@ -1,18 +1,24 @@
-// Tuned gfx950 kernels of the DG Euler residual and SSP-RK3 stage (FP64).
+// Tuned gfx950 kernels of the DG Euler residual and SSP-RK3 stage.
 //
-// k_vol_mfma<ORDER>: volume term on the FP64 matrix cores.  One wavefront owns
-//   16 consecutive elements.  Both contractions are v_mfma_f64_16x16x4 products:
+// Every kernel is a template on the contraction type T: T = double is the FP64 path
+// (the drop-in), T = float the mixed-precision path, which rounds coefficients and
+// tables to FP32 for the volume and face contractions (FP32 matrix cores / FP32 FMA,
+// FP32 accumulation) and keeps FP64 for the pointwise Euler and Riemann fluxes
+// (pressure, positivity clamps, wave speeds), the stage update and the state.
+//
+// k_vol_mfma<ORDER,T>: volume term on the matrix cores.  One wavefront owns
+//   16 consecutive elements.  Both contractions are 16x16x4 MFMA products:
 //     interpolation  U_v[q][e] = sum_k V[q][k] C_v[k][e]            (M=q, N=e, K=k)
 //     test           R_v[j][e] = sum_(q,d) G[q][d][j] B_v[(q,d)][e]   (M=j, N=e, K=(q,d))
 //   with B_v[(q,d)][e] = -w_q |J_e| (F(U_e(xi_q)) A_e^{-1})_(v,d), the pointwise
 //   Euler flux computed in FP64 by the lane that already holds U at that point
-//   (lane l: element l%16, points q = 16t + 4r + l/16, r = 0..3: the result
-//   layout of v_mfma_f64_16x16x4 puts row 4r + l/16, column l%16 in element r
-//   of lane l).  The K dimension of the test product is ordered so that lane group
-//   l/16 feeds exactly those points: no data movement between the two products.
-//   The 16-element coefficient tile (contiguous in the element-major layout) is
-//   staged once through LDS with full-width coalesced loads, and the result goes
-//   back the same way.  Accumulators live in AGPRs.
+//   (lane l: element l%16, the 4 points q = 16t + row(r, l/16), r = 0..3, where
+//   row() is the accumulator layout of the MFMA form: 4r + l/16 for FP64,
+//   4(l/16) + r for FP32).  The K dimension of the test product is ordered so that
+//   lane group l/16 feeds exactly those points: no data movement between the two
+//   products.  The 16-element coefficient tile (contiguous in the element-major
+//   layout) is staged once through LDS with full-width coalesced loads, and the
+//   result goes back the same way.  Accumulators live in AGPRs.
 //
 // Faces (no atomics anywhere; every output has one fixed summation order):
 //   k_face_flux_lds + k_face_test_lds  two passes (default): each face's traces and
@@ -24,21 +30,24 @@
 //   combination) when used inside the time stepper, so the total residual never goes
 //   to memory.
 #pragma once
+#include <type_traits>
+
 #include "fvdg_kernels.hpp"
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Cooperative global -> LDS copy of TOTAL doubles by NT threads: every load of the
+// Cooperative global -> LDS copy of TOTAL values by NT threads: every load of the
 // thread is issued before the first LDS store, so the block pays one memory
 // latency per staging round instead of one per element copied.
 template <int TOTAL, int NT, typename Src, typename Dst>
 __device__ __forceinline__ void stage_to_lds(int t, Src src, Dst dst) {
+    using V = decltype(src(0));
     constexpr int PF = (TOTAL + NT - 1) / NT;
-    double tmp[PF];
+    V tmp[PF];
 #pragma unroll
     for (int m = 0; m < PF; ++m) {
         const int i = t + m * NT;
-        tmp[m] = (i < TOTAL) ? src(i) : 0.0;
+        tmp[m] = (i < TOTAL) ? src(i) : V(0);
     }
 #pragma unroll
     for (int m = 0; m < PF; ++m) {
@@ -63,120 +72,176 @@ struct VolShape {
     static constexpr int KS = (N + 3) / 4;    // K steps of the interpolation product
     static constexpr int JT = (N + 15) / 16;  // test-function tiles of the test product
     static constexpr int QPAD = 16 * QT, KPAD = 4 * KS, JPAD = 16 * JT;
-    static constexpr int STRIDE = NV | 1;     // odd LDS row stride (doubles): spreads banks
+    static constexpr int STRIDE = NV | 1;     // odd LDS row stride: spreads banks
 };
 
+// The 16x16x4 matrix-core product in the contraction type T.  Operand lanes are the
+// same for both forms (A: row l%16, k l/16; B: k l/16, column l%16); the accumulator
+// element r of lane l holds column l%16 and row `row(r, l/16)`, which differs.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static constexpr int row(int r, int grp) { return 4 * r + grp; }
+};
+template <>
+struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static constexpr int row(int r, int grp) { return 4 * grp + r; }
+};
+
+template <typename T>
 struct VolArgs {
     int64_t E;
     double gamma;
-    const double* U;      // E*NV
-    double* R;            // E*NV (written)
-    const double* cgeo;   // E*10
-    const double* VA;     // QPAD*KPAD   V[q][k], zero padded
-    const double* GA;     // QPAD*3*JPAD G[q][d][j], zero padded
-    const double* wq;     // QPAD, zero padded
+    const double* U;     // E*NV state (FP64)
+    double* R;           // E*NV volume residual (written, FP64)
+    const double* cgeo;  // E*10
+    const T* VA;         // QPAD*KPAD   V[q][k], zero padded
+    const T* GA;         // QPAD*3*JPAD G[q][d][j], zero padded
+    const double* wq;    // QPAD, zero padded
+    const double* Gs;    // N*3: sum_q w_q d phi_j / d xi_d (FP32 path only)
 };
 
-template <int ORDER>
-__global__ __launch_bounds__(64) void k_vol_mfma(VolArgs a) {
+// k_vol_mfma<ORDER,T>: T = double is the FP64 path.  T = float splits off the parts
+// that cancel against the face terms and keeps them in FP64: the interpolation adds
+// the mode-0 coefficient (phi_0 = 1) in FP64 to an FP32 product over modes >= 1, and
+// the test product runs on F(U_q) - F(c_0) (c_0 = cell mean) in FP32, while the mean
+// flux's exact contribution -|J| (F(c_0) A^{-1}) . sum_q w_q grad phi_j is added in
+// FP64.  FP32 rounding then scales with the variation of the state inside the cell,
+// and a uniform state has an FP64-exact residual.
+template <int ORDER, typename T>
+__global__ __launch_bounds__(64) void k_vol_mfma(VolArgs<T> a) {
     using S = VolShape<ORDER>;
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
     constexpr int N = S::N, Q = S::Q, NV = S::NV, QT = S::QT, KS = S::KS, JT = S::JT;
     constexpr int KPAD = S::KPAD, JPAD = S::JPAD, STR = S::STRIDE;
-    __shared__ double sC[16 * STR];
+    __shared__ T sC[16 * STR];
+    __shared__ double sOut[SPLIT ? 16 * STR : 1];
     const int l = threadIdx.x, col = l & 15, grp = l >> 4;
     const int64_t e0 = (int64_t)blockIdx.x * 16;
     const int64_t base = e0 * NV, limit = a.E * NV;
 
-    // 1. stage the tile (16*NV contiguous doubles) in LDS
+    // 1. stage the tile (16*NV contiguous doubles) in LDS (split: mode 0 left out)
     stage_to_lds<16 * NV, 64>(
-        l, [&](int i) { return (base + i < limit) ? a.U[base + i] : 0.0; },
-        [&](int i, double x) { sC[(i / NV) * STR + (i % NV)] = x; });
+        l,
+        [&](int i) { return (base + i < limit && (!SPLIT || i % NV >= 5)) ? (T)a.U[base + i] : (T)0; },
+        [&](int i, T x) { sC[(i / NV) * STR + (i % NV)] = x; });
     __syncthreads();
 
-    // 2. geometry of this lane's element (zeros past the end)
+    // 2. geometry of this lane's element (zeros past the end); split: mean state and flux
     const int64_t e = e0 + col;
-    double Ai[9], det = 0.0;
+    double Ai[9], det = 0.0, c0[5] = {0, 0, 0, 0, 0}, FrA[5][3];
     if (e < a.E) {
         const double* g = a.cgeo + e * 10;
 #pragma unroll
         for (int i = 0; i < 9; ++i) Ai[i] = g[i];
         det = g[9];
+        if (SPLIT)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) c0[v] = a.U[e * NV + v];
     } else {
 #pragma unroll
         for (int i = 0; i < 9; ++i) Ai[i] = 0.0;
     }
+    if (SPLIT) {
+        double F[5][3];
+        euler_flux(c0, a.gamma, F);
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) FrA[v][d] = F[v][0] * Ai[d] + F[v][1] * Ai[3 + d] + F[v][2] * Ai[6 + d];
+    }
 
-    v4d racc[5][JT];
+    acc_t racc[5][JT];
 #pragma unroll
     for (int v = 0; v < 5; ++v)
 #pragma unroll
-        for (int jt = 0; jt < JT; ++jt) racc[v][jt] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int jt = 0; jt < JT; ++jt) racc[v][jt] = (acc_t){0, 0, 0, 0};
 
 #pragma unroll 1
     for (int t = 0; t < QT; ++t) {
         // 3. interpolation to the 16 points of tile t
-        v4d uacc[5];
+        acc_t uacc[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) uacc[v] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int v = 0; v < 5; ++v) uacc[v] = (acc_t){0, 0, 0, 0};
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            const double av = a.VA[(16 * t + col) * KPAD + 4 * s + grp];
+            const T av = a.VA[(16 * t + col) * KPAD + 4 * s + grp];
             const int k = 4 * s + grp;
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                const double bv = (k < N) ? sC[col * STR + 5 * k + v] : 0.0;
-                uacc[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, uacc[v], 0, 0, 0);
+                const T bv = (k < N) ? sC[col * STR + 5 * k + v] : (T)0;
+                uacc[v] = M::mma(av, bv, uacc[v]);
             }
         }
         // 4. pointwise flux at this lane's 4 points, fed straight into the test product
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int q = 16 * t + 4 * r + grp;
-            double B[5][3];
+            const int q = 16 * t + M::row(r, grp);
+            T B[5][3];
             if (q < Q) {
-                const double Uq[5] = {uacc[0][r], uacc[1][r], uacc[2][r], uacc[3][r], uacc[4][r]};
+                double Uq[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Uq[v] = c0[v] + (double)uacc[v][r];
                 double F[5][3];
                 euler_flux(Uq, a.gamma, F);
                 const double jw = -a.wq[q] * det;
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        B[v][d] = (F[v][0] * Ai[d] + F[v][1] * Ai[3 + d] + F[v][2] * Ai[6 + d]) * jw;
+                    for (int d = 0; d < 3; ++d) {
+                        double fa = F[v][0] * Ai[d] + F[v][1] * Ai[3 + d] + F[v][2] * Ai[6 + d];
+                        if (SPLIT) fa -= FrA[v][d];
+                        B[v][d] = (T)(fa * jw);
+                    }
             } else {
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) B[v][d] = 0.0;
+                    for (int d = 0; d < 3; ++d) B[v][d] = (T)0;
             }
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt) {
-                    const double av = a.GA[(q * 3 + d) * JPAD + 16 * jt + col];
+                    const T av = a.GA[(q * 3 + d) * JPAD + 16 * jt + col];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v)
-                        racc[v][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, B[v][d], racc[v][jt], 0, 0, 0);
+                    for (int v = 0; v < 5; ++v) racc[v][jt] = M::mma(av, B[v][d], racc[v][jt]);
                 }
             }
         }
     }
 
-    // 5. result R_v[j = 16 jt + 4 r + grp][element col] -> LDS -> coalesced store
+    // 5. result R_v[j = 16 jt + row(r, grp)][element col] -> LDS -> coalesced store
     __syncthreads();
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int j = 16 * jt + 4 * r + grp;
-            if (j < N)
+            const int j = 16 * jt + M::row(r, grp);
+            if (j < N) {
+                if (SPLIT) {
+                    const double g0 = a.Gs[3 * j], g1 = a.Gs[3 * j + 1], g2 = a.Gs[3 * j + 2];
 #pragma unroll
-                for (int v = 0; v < 5; ++v) sC[col * STR + 5 * j + v] = racc[v][jt][r];
+                    for (int v = 0; v < 5; ++v)
+                        sOut[col * STR + 5 * j + v] =
+                            (double)racc[v][jt][r] - det * (FrA[v][0] * g0 + FrA[v][1] * g1 + FrA[v][2] * g2);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) sC[col * STR + 5 * j + v] = racc[v][jt][r];
+                }
+            }
         }
     __syncthreads();
     for (int i = l; i < 16 * NV; i += 64) {
         const int ee = i / NV, kv = i - ee * NV;
-        if (base + i < limit) a.R[base + i] = sC[ee * STR + kv];
+        if (base + i < limit) a.R[base + i] = SPLIT ? sOut[ee * STR + kv] : (double)sC[ee * STR + kv];
     }
 }
 
@@ -426,13 +491,22 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
 }
 
 // ------------------------------------------------------------------ faces in two passes
-// k_face_flux_lds<ORDER>: every face once (face-based, no duplicated work): lanes =
+// k_face_flux_lds<ORDER,T>: every face once (face-based, no duplicated work): lanes =
 //   (face, 2 face points); both traces from LDS-staged coefficient blocks and the
-//   canonical LDS tables, ghost state on boundary faces, LF/Roe flux, scaled by
-//   w_g * 2A_f and written to Fh[f][g][v] (contiguous per face).
-// k_face_test_lds<ORDER,FUSE>: lanes = (element, test function j); each local face's
+//   canonical LDS tables (contraction in T), ghost state on boundary faces, LF/Roe
+//   flux in FP64, scaled by w_g * 2A_f and written to Fh[f][g][v] (type T,
+//   contiguous per face).
+// k_face_test_lds<ORDER,FUSE,T>: lanes = (element, test function j); each local face's
 //   Fh of the tile's elements is staged in LDS with the side's sign, then
-//   sum_faces sum_g phi_j(xi_side(g)) Fh[g] is added to R or fed to the SSP-RK stage.
+//   sum_faces sum_g phi_j(xi_side(g)) Fh[g] (in T) is added to the volume part Rv
+//   and written to R (FP64), or fed to the SSP-RK stage update (FP64).
+// With T = float the parts that cancel between neighbours and against the volume term
+// stay FP64: traces add the mode-0 coefficient in FP64 to an FP32 product over modes
+// >= 1, and each face stores a reference flux Fref (its point-0 flux, FP64, times 2A)
+// plus the FP32 differences Fh[g] = (Fhat_g - Fref) w_g 2A; the test pass adds
+// Fref 2A sum_g w_g phi_j(xi(g)) in FP64 (sum_g w_g phi_j depends only on the
+// canonical face table, table IC).  A uniform flux then cancels exactly in FP64.
+template <typename T>
 struct FaceFluxArgs {
     int64_t F;
     double gamma;
@@ -445,10 +519,11 @@ struct FaceFluxArgs {
     const int32_t* fdir;
     const double* UD;
     const double* wf;
-    const double* PhiC;
+    const T* PhiC;
     const uint8_t* canon;
     const int32_t* perm;
-    double* Fh;  // F*QF*5
+    T* Fh;         // F*QF*5
+    double* Fref;  // F*5 (FP32 path only)
     int64_t ntiles;
 };
 
@@ -461,13 +536,15 @@ struct FaceFluxShape {
     static constexpr int FB = THREADS / LPF;                // faces per tile
 };
 
-template <int ORDER>
-__global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs a) {
+template <int ORDER, typename T>
+__global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs<T> a) {
     using SH = FaceFluxShape<ORDER>;
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
     constexpr int N = SH::N, QF = SH::QF, NV = 5 * N, GP = SH::GP, LPF = SH::LPF, FB = SH::FB;
     constexpr int NT = SH::THREADS, NS = N | 1;  // odd table row stride: spreads LDS banks
-    __shared__ double sPhi[8 * QF * NS];
-    __shared__ double sC[2 * FB * NV];
+    __shared__ T sPhi[8 * QF * NS];
+    __shared__ T sC[2 * FB * NV];
+    __shared__ double sC0[SPLIT ? 2 * FB * 5 : 1], sRef[SPLIT ? FB * 5 : 1];
     __shared__ int sPerm[3 * QF];
     __shared__ int sCanon[24];
     __shared__ int sCell[2 * FB], sCb[2 * FB], sKind[FB];
@@ -476,6 +553,7 @@ __global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs a) {
     for (int i = t; i < 8 * QF * N; i += NT) sPhi[(i / N) * NS + (i % N)] = a.PhiC[i];
     for (int i = t; i < 3 * QF; i += NT) sPerm[i] = a.perm[i];
     if (t < 24) sCanon[t] = a.canon[t];
+    const int fl = t / LPF, pg = t - fl * LPF;
 
     for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const int64_t f0 = tile * FB;
@@ -504,86 +582,130 @@ __global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs a) {
             t,
             [&](int i) {
                 const int c = sCell[i / NV];
-                return c >= 0 ? a.U[(int64_t)c * NV + (i % NV)] : 0.0;
+                return (c >= 0 && (!SPLIT || i % NV >= 5)) ? (T)a.U[(int64_t)c * NV + (i % NV)] : (T)0;
             },
-            [&](int i, double x) { sC[i] = x; });
+            [&](int i, T x) { sC[i] = x; });
+        if (SPLIT)
+            for (int i = t; i < 2 * FB * 5; i += NT) {
+                const int c = sCell[i / 5];
+                sC0[i] = c >= 0 ? a.U[(int64_t)c * NV + i % 5] : 0.0;
+            }
         __syncthreads();
-        if (t < FB * LPF) {
-            const int fl = t / LPF, pg = t - fl * LPF;
-            const int64_t f = f0 + fl;
+        const int64_t f = f0 + fl;
+        const bool lane_on = t < FB * LPF && f < a.F;
+        double fo[GP][5];
+        double J = 0.0;
+        if (lane_on) {
             const int kind = sKind[fl];
-            if (f < a.F) {
-                double UL[GP][5], UR[GP][5];
-                int gl[GP], gr[GP];
-                const int cL = sCb[2 * fl], cR = sCb[2 * fl + 1];
-                const double* PL = sPhi + (cL / 3) * QF * NS;
-                const double* PR = sPhi + (cR / 3) * QF * NS;
+            T UL[GP][5], UR[GP][5];
+            int gl[GP], gr[GP];
+            const int cL = sCb[2 * fl], cR = sCb[2 * fl + 1];
+            const T* PL = sPhi + (cL / 3) * QF * NS;
+            const T* PR = sPhi + (cR / 3) * QF * NS;
 #pragma unroll
-                for (int i = 0; i < GP; ++i) {
-                    const int g = min(pg * GP + i, QF - 1);
-                    gl[i] = sPerm[(cL % 3) * QF + g];
-                    gr[i] = sPerm[(cR % 3) * QF + g];
+            for (int i = 0; i < GP; ++i) {
+                const int g = min(pg * GP + i, QF - 1);
+                gl[i] = sPerm[(cL % 3) * QF + g];
+                gr[i] = sPerm[(cR % 3) * QF + g];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) UL[i][v] = UR[i][v] = 0.0;
-                }
-                if (kind != 0) {
-                    const double* c0 = sC + (2 * fl) * NV;
-                    const double* c1 = sC + (2 * fl + 1) * NV;
+                for (int v = 0; v < 5; ++v) UL[i][v] = UR[i][v] = (T)0;
+            }
+            if (kind != 0) {
+                const T* c0 = sC + (2 * fl) * NV;
+                const T* c1 = sC + (2 * fl + 1) * NV;
 #pragma unroll 4
-                    for (int k = 0; k < N; ++k) {
-                        double c[5], d[5];
+                for (int k = SPLIT ? 1 : 0; k < N; ++k) {
+                    T c[5], d[5];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        c[v] = c0[5 * k + v];
+                        d[v] = c1[5 * k + v];
+                    }
+#pragma unroll
+                    for (int i = 0; i < GP; ++i) {
+                        const T pl = PL[gl[i] * NS + k], pr = PR[gr[i] * NS + k];
 #pragma unroll
                         for (int v = 0; v < 5; ++v) {
-                            c[v] = c0[5 * k + v];
-                            d[v] = c1[5 * k + v];
-                        }
-#pragma unroll
-                        for (int i = 0; i < GP; ++i) {
-                            const double pl = PL[gl[i] * NS + k], pr = PR[gr[i] * NS + k];
-#pragma unroll
-                            for (int v = 0; v < 5; ++v) {
-                                UL[i][v] += pl * c[v];
-                                UR[i][v] += pr * d[v];
-                            }
+                            UL[i][v] += pl * c[v];
+                            UR[i][v] += pr * d[v];
                         }
                     }
                 }
-                const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
-                const double J = a.fgeo[4 * f + 3];
+            }
+            const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
+            J = a.fgeo[4 * f + 3];
 #pragma unroll
-                for (int i = 0; i < GP; ++i) {
-                    const int g = pg * GP + i;
-                    if (g >= QF) continue;
-                    double out[5] = {0, 0, 0, 0, 0};
-                    if (kind == 1) {
-                        num_flux(a.flux, UL[i], UR[i], n, a.gamma, out);
-                    } else if (kind == 2) {
-                        double Ug[5];
-                        if (a.fbc[f] == FVDG_BC_DIRICHLET) {
-                            const double* ud = a.UD + ((int64_t)a.fdir[f] * QF + g) * 5;
+            for (int i = 0; i < GP; ++i) {
+                const int g = pg * GP + i;
+                double uL[5], uR[5];
 #pragma unroll
-                            for (int v = 0; v < 5; ++v) Ug[v] = ud[v] + (ud[v] - UL[i][v]);
-                        } else {
-#pragma unroll
-                            for (int v = 0; v < 5; ++v) Ug[v] = UL[i][v];
-                            Ug[3] = -UL[i][3];
-                        }
-                        num_flux(a.flux, UL[i], Ug, n, a.gamma, out);
-                    }
-                    const double sc = a.wf[g] * J;
-                    double* o = a.Fh + (f * QF + g) * 5;
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) o[v] = out[v] * sc;
+                for (int v = 0; v < 5; ++v) {
+                    fo[i][v] = 0.0;
+                    uL[v] = (SPLIT ? sC0[(2 * fl) * 5 + v] : 0.0) + (double)UL[i][v];
+                    uR[v] = (SPLIT ? sC0[(2 * fl + 1) * 5 + v] : 0.0) + (double)UR[i][v];
                 }
+                if (g >= QF) continue;
+                if (kind == 1) {
+                    num_flux(a.flux, uL, uR, n, a.gamma, fo[i]);
+                } else if (kind == 2) {
+                    double Ug[5];
+                    if (a.fbc[f] == FVDG_BC_DIRICHLET) {
+                        const double* ud = a.UD + ((int64_t)a.fdir[f] * QF + g) * 5;
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) Ug[v] = ud[v] + (ud[v] - uL[v]);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) Ug[v] = uL[v];
+                        Ug[3] = -uL[3];
+                    }
+                    num_flux(a.flux, uL, Ug, n, a.gamma, fo[i]);
+                }
+            }
+        }
+        if (SPLIT) {
+            // the face's reference flux: its point 0, held by lane pg = 0 as i = 0
+            if (lane_on && pg == 0)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sRef[fl * 5 + v] = fo[0][v];
+            __syncthreads();
+            if (lane_on && pg == 0)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) a.Fref[f * 5 + v] = fo[0][v] * J;
+        }
+        if (lane_on) {
+#pragma unroll
+            for (int i = 0; i < GP; ++i) {
+                const int g = pg * GP + i;
+                if (g >= QF) continue;
+                const double sc = a.wf[g] * J;
+                T* o = a.Fh + (f * QF + g) * 5;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) o[v] = (T)((fo[i][v] - (SPLIT ? sRef[fl * 5 + v] : 0.0)) * sc);
             }
         }
     }
 }
 
+template <typename T>
 struct FaceTestArgs {
-    SurfArgs s;  // U, R, cgeo, cfaces, fcell, fcombo, parts, accumulate, fused-update fields
-    const double* Fh;
-    const double* PhiC;
+    int64_t E;
+    int parts;
+    const double* U;      // stage input state
+    const double* Rv;     // volume part of the residual (nullable: none)
+    double* R;            // residual output (FUSE = false)
+    const double* cgeo;
+    const int32_t* cfaces;
+    const int32_t* fcell;
+    const uint8_t* fcombo;
+    // fused SSP-RK stage: out = a0*U0 + b0*(U + dt*(-R/M)), M = |det| * mhat[k]
+    const double* U0;
+    double* out;
+    const double* mhat;
+    double a0, b0, dt;
+    const T* Fh;
+    const double* Fref;   // F*5 (FP32 path only)
+    const double* IC;     // 8*N: sum_g w_g PhiC[c][g][j] (FP32 path only)
+    const T* PhiC;
     const uint8_t* canon;
     const int32_t* perm;
     int64_t ntiles;
@@ -596,27 +718,30 @@ struct FaceTestShape {
     static constexpr int TE = THREADS / N > 32 ? 32 : THREADS / N;  // elements per tile
 };
 
-// k_face_test_lds<ORDER,FUSE>: the test pass.  The four local faces are staged one
-// round at a time (TE*Qf*5 doubles of LDS): measured faster than staging all four
+// k_face_test_lds<ORDER,FUSE,T>: the test pass.  The four local faces are staged one
+// round at a time (TE*Qf*5 values of LDS): measured faster than staging all four
 // at once (smaller footprint, more blocks per compute unit).
-template <int ORDER, bool FUSE>
-__global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs A) {
+template <int ORDER, bool FUSE, typename T>
+__global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs<T> a) {
     using SH = FaceTestShape<ORDER>;
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
     constexpr int N = SH::N, QF = SH::QF, NV = 5 * N, TE = SH::TE, NT = SH::THREADS;
-    const SurfArgs& a = A.s;
-    __shared__ double sPhi[8 * QF * N];
-    __shared__ double sF[TE * QF * 5];
+    __shared__ T sPhi[8 * QF * N];
+    __shared__ T sF[TE * QF * 5];
+    __shared__ double sIC[SPLIT ? 8 * N : 1], sRefF[SPLIT ? TE * 5 : 1];
     __shared__ int sPerm[3 * QF];
     __shared__ int sCanon[24];
     __shared__ int sFace[TE * 4], sCb[TE * 4];
 
     const int t = threadIdx.x;
-    for (int i = t; i < 8 * QF * N; i += NT) sPhi[i] = A.PhiC[i];
-    for (int i = t; i < 3 * QF; i += NT) sPerm[i] = A.perm[i];
-    if (t < 24) sCanon[t] = A.canon[t];
+    for (int i = t; i < 8 * QF * N; i += NT) sPhi[i] = a.PhiC[i];
+    for (int i = t; i < 3 * QF; i += NT) sPerm[i] = a.perm[i];
+    if (SPLIT)
+        for (int i = t; i < 8 * N; i += NT) sIC[i] = a.IC[i];
+    if (t < 24) sCanon[t] = a.canon[t];
     const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
 
-    for (int64_t tile = blockIdx.x; tile < A.ntiles; tile += gridDim.x) {
+    for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const int64_t e_base = tile * TE;
         __syncthreads();
         if (t < TE * 4) {
@@ -634,7 +759,8 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs A) {
         }
         const int el = t / N, j = t - el * N;
         const bool active = t < TE * N && e_base + el < a.E;
-        double r[5] = {0, 0, 0, 0, 0};
+        T r[5] = {0, 0, 0, 0, 0};
+        double rc[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
         for (int lf = 0; lf < 4; ++lf) {
             __syncthreads();
@@ -643,22 +769,37 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs A) {
                 [&](int i) {
                     const int e2 = i / (QF * 5), gv = i - e2 * QF * 5;
                     const int slot = sFace[e2 * 4 + lf];
-                    if (slot < 0) return 0.0;
-                    const double x = A.Fh[(int64_t)(slot >> 1) * QF * 5 + gv];
+                    if (slot < 0) return (T)0;
+                    const T x = a.Fh[(int64_t)(slot >> 1) * QF * 5 + gv];
                     return (slot & 1) ? -x : x;
                 },
-                [&](int i, double x) { sF[i] = x; });
+                [&](int i, T x) { sF[i] = x; });
+            if (SPLIT)
+                for (int i = t; i < TE * 5; i += NT) {
+                    const int slot = sFace[(i / 5) * 4 + lf];
+                    double x = 0.0;
+                    if (slot >= 0) {
+                        x = a.Fref[(int64_t)(slot >> 1) * 5 + i % 5];
+                        if (slot & 1) x = -x;
+                    }
+                    sRefF[i] = x;
+                }
             __syncthreads();
             if (active && sFace[el * 4 + lf] >= 0) {
                 const int cb = sCb[el * 4 + lf];
-                const double* P = sPhi + (cb / 3) * QF * N + j;
+                const T* P = sPhi + (cb / 3) * QF * N + j;
                 const int* pm = sPerm + (cb % 3) * QF;
-                const double* fh = sF + el * QF * 5;
+                const T* fh = sF + el * QF * 5;
 #pragma unroll 4
                 for (int g = 0; g < QF; ++g) {
-                    const double p = P[pm[g] * N];
+                    const T p = P[pm[g] * N];
 #pragma unroll
                     for (int v = 0; v < 5; ++v) r[v] += p * fh[g * 5 + v];
+                }
+                if (SPLIT) {
+                    const double ic = sIC[(cb / 3) * N + j];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) rc[v] += ic * sRefF[el * 5 + v];
                 }
             }
         }
@@ -669,13 +810,16 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs A) {
                 const double M = a.cgeo[e * 10 + 9] * a.mhat[j];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    const double Rt = a.R[i0 + v] + r[v];
+                    const double Rt = a.Rv[i0 + v] + (rc[v] + (double)r[v]);
                     const double sv = a.U[i0 + v] + a.dt * (-Rt / M);
                     a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
                 }
             } else {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) a.R[i0 + v] = a.accumulate ? a.R[i0 + v] + r[v] : r[v];
+                for (int v = 0; v < 5; ++v) {
+                    const double rf = rc[v] + (double)r[v];
+                    a.R[i0 + v] = a.Rv ? a.Rv[i0 + v] + rf : rf;
+                }
             }
         }
     }

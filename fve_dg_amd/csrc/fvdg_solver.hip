@@ -91,12 +91,16 @@ struct fvdg_ctx {
     DevBuf<double> tV, tG, tWq, tPhi, tWf, tMhat;
     DevBuf<double> tVA, tGA, tWqa, tPhiT;  // zero-padded MFMA operand tables, transposed face table
     DevBuf<double> tPhiC;                 // canonical face tables (8 vertex orders)
+    DevBuf<float> tVA32, tGA32, tPhiC32;  // FP32 copies for the mixed-precision kernels
+    DevBuf<double> tGs, tIC;              // mixed: sum_q w_q grad phi_j (N*3), sum_g w_g PhiC (8*N)
     DevBuf<uint8_t> tCanon;               // per combo: canonical index * 3 + rotation
     DevBuf<int32_t> tPerm;                // face-rule point permutations of the 3 rotations
     int n_cu = 256;
     // device state: U (state), Ua/Ub (stage ping-pong), R (residual)
     DevBuf<double> U, Ua, Ub, R;
     DevBuf<double> Fq, Fh;           // Fq: simple-kernel scratch E*Q*15; Fh: face fluxes F*Qf*5
+    DevBuf<float> Fh32;              // mixed precision: face flux differences F*Qf*5
+    DevBuf<double> Fref;             // mixed precision: per-face reference flux F*5
     bool simple = false;             // FVDG_KERNELS=simple
     bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
     DevBuf<double> red;              // reduction partials
@@ -154,8 +158,34 @@ int upload_tables(fvdg_ctx* c) {
         return fail(FVDG_ERR_UNSUPPORTED, "face rule is not invariant under vertex rotation (deviation " +
                                               std::to_string(dev) + ")");
     HIP_TRY(c->tPhiC.upload(PC, c->stream));
+    HIP_TRY(c->tVA32.upload(std::vector<float>(VA.begin(), VA.end()), c->stream));
+    HIP_TRY(c->tGA32.upload(std::vector<float>(GA.begin(), GA.end()), c->stream));
+    HIP_TRY(c->tPhiC32.upload(std::vector<float>(PC.begin(), PC.end()), c->stream));
+    std::vector<double> Gs(T.N * 3, 0.0), IC(8 * T.N, 0.0);
+    for (int j = 0; j < T.N; ++j)
+        for (int d = 0; d < 3; ++d)
+            for (int q = 0; q < T.Q; ++q) Gs[j * 3 + d] += T.vq_w[q] * T.G[(q * 3 + d) * T.N + j];
+    for (int cc = 0; cc < 8; ++cc)
+        for (int j = 0; j < T.N; ++j)
+            for (int g = 0; g < T.Qf; ++g) IC[cc * T.N + j] += T.fq_w[g] * PC[((size_t)cc * T.Qf + g) * T.N + j];
+    HIP_TRY(c->tGs.upload(Gs, c->stream));
+    HIP_TRY(c->tIC.upload(IC, c->stream));
     HIP_TRY(c->tCanon.upload(canon, c->stream));
     HIP_TRY(c->tPerm.upload(perm, c->stream));
+    return FVDG_OK;
+}
+
+// Scratch buffers of the contraction precision (face fluxes, and for the mixed path the
+// FP32 volume residual); the state buffers are not touched.
+int precision_buffers(fvdg_ctx* c) {
+    if (c->order < 0 || !c->have_mesh) return FVDG_OK;
+    const size_t nfh = (size_t)c->F * c->tab.Qf * 5;
+    if (c->precision == FVDG_PREC_FP64 || c->simple) {
+        if (c->Fh.n != nfh) HIP_TRY(c->Fh.alloc(nfh));
+    } else {
+        if (c->Fh32.n != nfh) HIP_TRY(c->Fh32.alloc(nfh));
+        if (c->Fref.n != (size_t)c->F * 5) HIP_TRY(c->Fref.alloc((size_t)c->F * 5));
+    }
     return FVDG_OK;
 }
 
@@ -170,7 +200,8 @@ int alloc_state(fvdg_ctx* c) {
     for (DevBuf<double>* b : {&c->U, &c->Ua, &c->Ub, &c->R})
         HIP_TRY(hipMemsetAsync(b->p, 0, ndof * sizeof(double), c->stream));
     if (c->simple) HIP_TRY(c->Fq.alloc((size_t)c->E * c->tab.Q * 15));
-    HIP_TRY(c->Fh.alloc((size_t)c->F * c->tab.Qf * 5));
+    int rc = precision_buffers(c);
+    if (rc) return rc;
     HIP_TRY(c->UD.alloc((size_t)std::max<int64_t>(c->n_dir, 1) * c->tab.Qf * 5));
     HIP_TRY(hipMemsetAsync(c->UD.p, 0, c->UD.n * sizeof(double), c->stream));
     c->n_red = std::max<int64_t>(nblocks(std::max(c->E, c->F * c->tab.Qf), 256), 1);
@@ -264,11 +295,40 @@ struct Span {
     }
 };
 
-template <int P>
-void launch_vol_fast(fvdg_ctx* c, const double* U, double* R) {
-    VolArgs v{c->E, c->gamma, U, R, c->cgeo.p, c->tVA.p, c->tGA.p, c->tWqa.p};
-    hipLaunchKernelGGL(k_vol_mfma<P>, dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, v);
+// SSP-RK stage combination applied by the last face kernel: out = a0*U0 + b0*(U + dt*(-R/M))
+struct StageUpdate {
+    const double* U0;
+    double* out;
+    double a0, b0, dt;
+};
+
+// the FP32-contraction kernels (the plain cross-check kernels are FP64 only)
+inline bool mixed(const fvdg_ctx* c) { return c->precision != FVDG_PREC_FP64 && !c->simple; }
+
+template <int P, typename T>
+void launch_vol_t(fvdg_ctx* c, const double* U, double* R, const T* VA, const T* GA) {
+    VolArgs<T> v{c->E, c->gamma, U, R, c->cgeo.p, VA, GA, c->tWqa.p, c->tGs.p};
+    hipLaunchKernelGGL((k_vol_mfma<P, T>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, v);
 }
+template <int P>
+void launch_vol_fast(fvdg_ctx* c, const double* U) {
+    if (mixed(c))
+        launch_vol_t<P, float>(c, U, c->R.p, c->tVA32.p, c->tGA32.p);
+    else
+        launch_vol_t<P, double>(c, U, c->R.p, c->tVA.p, c->tGA.p);
+}
+// volume part of the residual of U into c->R
+void dispatch_vol(fvdg_ctx* c, const double* U) {
+    switch (c->order) {
+        case 0: launch_vol_fast<0>(c, U); break;
+        case 1: launch_vol_fast<1>(c, U); break;
+        case 2: launch_vol_fast<2>(c, U); break;
+        case 3: launch_vol_fast<3>(c, U); break;
+        case 4: launch_vol_fast<4>(c, U); break;
+        case 5: launch_vol_fast<5>(c, U); break;
+    }
+}
+
 template <int P>
 void launch_surf_fast(fvdg_ctx* c, const SurfArgs& a, bool fuse) {
     constexpr int TE = SurfLdsShape<P>::TE, NT = SurfLdsShape<P>::THREADS;
@@ -280,68 +340,100 @@ void launch_surf_fast(fvdg_ctx* c, const SurfArgs& a, bool fuse) {
     else
         hipLaunchKernelGGL((k_surf_lds<P, false>), dim3(grid), dim3(NT), 0, c->stream, A);
 }
-void dispatch_vol(fvdg_ctx* c, const double* U, double* R) {
-    switch (c->order) {
-        case 0: launch_vol_fast<0>(c, U, R); break;
-        case 1: launch_vol_fast<1>(c, U, R); break;
-        case 2: launch_vol_fast<2>(c, U, R); break;
-        case 3: launch_vol_fast<3>(c, U, R); break;
-        case 4: launch_vol_fast<4>(c, U, R); break;
-        case 5: launch_vol_fast<5>(c, U, R); break;
-    }
-}
-template <int P>
-void launch_faces_two_pass(fvdg_ctx* c, const SurfArgs& a, bool fuse, double* ms_flux, double* ms_test) {
-    FaceFluxArgs f{c->F, c->gamma, c->flux, a.U, c->fcell.p, c->fcombo.p, c->fgeo.p, c->fbc.p, c->fdir.p, c->UD.p,
-                   c->tWf.p, c->tPhiC.p, c->tCanon.p, c->tPerm.p, c->Fh.p, 0};
+
+template <int P, typename T>
+void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, double* Rout, const StageUpdate* up,
+                    const T* PhiC, T* Fh, double* ms_flux, double* ms_test) {
+    FaceFluxArgs<T> f{c->F, c->gamma, c->flux, U, c->fcell.p, c->fcombo.p, c->fgeo.p, c->fbc.p, c->fdir.p,
+                      c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0};
     constexpr int FB = FaceFluxShape<P>::FB;
     f.ntiles = (c->F + FB - 1) / FB;
     const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * 8);
     Span s1(ms_flux, c->stream);
-    hipLaunchKernelGGL(k_face_flux_lds<P>, dim3(g1), dim3(FaceFluxShape<P>::THREADS), 0, c->stream, f);
+    hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P>::THREADS), 0, c->stream, f);
     s1.stop();
-    Span s2(ms_test, c->stream);
     constexpr int TE = FaceTestShape<P>::TE;
-    FaceTestArgs t{a, c->Fh.p, c->tPhiC.p, c->tCanon.p, c->tPerm.p, (c->E + TE - 1) / TE};
+    FaceTestArgs<T> t{};
+    t.E = c->E;
+    t.parts = parts;
+    t.U = U;
+    t.Rv = Rv;
+    t.R = Rout;
+    t.cgeo = c->cgeo.p;
+    t.cfaces = c->cfaces.p;
+    t.fcell = c->fcell.p;
+    t.fcombo = c->fcombo.p;
+    t.mhat = c->tMhat.p;
+    if (up) {
+        t.U0 = up->U0;
+        t.out = up->out;
+        t.a0 = up->a0;
+        t.b0 = up->b0;
+        t.dt = up->dt;
+    }
+    t.Fh = Fh;
+    t.Fref = c->Fref.p;
+    t.IC = c->tIC.p;
+    t.PhiC = PhiC;
+    t.canon = c->tCanon.p;
+    t.perm = c->tPerm.p;
+    t.ntiles = (c->E + TE - 1) / TE;
     const unsigned g2 = (unsigned)std::min<int64_t>(t.ntiles, (int64_t)c->n_cu * 8);
-    if (fuse)
-        hipLaunchKernelGGL((k_face_test_lds<P, true>), dim3(g2), dim3(FaceTestShape<P>::THREADS), 0, c->stream, t);
+    Span s2(ms_test, c->stream);
+    if (up)
+        hipLaunchKernelGGL((k_face_test_lds<P, true, T>), dim3(g2), dim3(FaceTestShape<P>::THREADS), 0, c->stream, t);
     else
-        hipLaunchKernelGGL((k_face_test_lds<P, false>), dim3(g2), dim3(FaceTestShape<P>::THREADS), 0, c->stream, t);
+        hipLaunchKernelGGL((k_face_test_lds<P, false, T>), dim3(g2), dim3(FaceTestShape<P>::THREADS), 0, c->stream, t);
     s2.stop();
 }
 
-// ms_flux / ms_test (nullable): event-timed face-flux and face-test(+update) kernels;
-// the one-pass owner-computes kernel is accounted under ms_flux.
-void dispatch_surf(fvdg_ctx* c, const SurfArgs& a, bool fuse, double* ms_flux = nullptr, double* ms_test = nullptr) {
-    if (!c->owner_faces) {
-        switch (c->order) {
-            case 0: launch_faces_two_pass<0>(c, a, fuse, ms_flux, ms_test); return;
-            case 1: launch_faces_two_pass<1>(c, a, fuse, ms_flux, ms_test); return;
-            case 2: launch_faces_two_pass<2>(c, a, fuse, ms_flux, ms_test); return;
-            case 3: launch_faces_two_pass<3>(c, a, fuse, ms_flux, ms_test); return;
-            case 4: launch_faces_two_pass<4>(c, a, fuse, ms_flux, ms_test); return;
-            case 5: launch_faces_two_pass<5>(c, a, fuse, ms_flux, ms_test); return;
+template <int P>
+void faces_fast(fvdg_ctx* c, const double* U, int parts, bool have_vol, const StageUpdate* up, double* ms_flux,
+                double* ms_test) {
+    if (mixed(c)) {
+        faces_two_pass<P, float>(c, U, parts, have_vol ? c->R.p : nullptr, c->R.p, up, c->tPhiC32.p, c->Fh32.p,
+                                 ms_flux, ms_test);
+    } else if (!c->owner_faces) {
+        faces_two_pass<P, double>(c, U, parts, have_vol ? c->R.p : nullptr, c->R.p, up, c->tPhiC.p, c->Fh.p, ms_flux,
+                                  ms_test);
+    } else {
+        SurfArgs a = surf_args(c, U, c->R.p, parts);
+        a.accumulate = have_vol;
+        if (up) {
+            a.U0 = up->U0;
+            a.out = up->out;
+            a.a0 = up->a0;
+            a.b0 = up->b0;
+            a.dt = up->dt;
         }
+        Span so(ms_flux, c->stream);
+        launch_surf_fast<P>(c, a, up != nullptr);
+        so.stop();
     }
-    Span so(ms_flux, c->stream);
-    switch (c->order) {
-        case 0: launch_surf_fast<0>(c, a, fuse); break;
-        case 1: launch_surf_fast<1>(c, a, fuse); break;
-        case 2: launch_surf_fast<2>(c, a, fuse); break;
-        case 3: launch_surf_fast<3>(c, a, fuse); break;
-        case 4: launch_surf_fast<4>(c, a, fuse); break;
-        case 5: launch_surf_fast<5>(c, a, fuse); break;
-    }
-    so.stop();
 }
 
-// R = selected parts of the residual of U (device buffers)
-int launch_residual(fvdg_ctx* c, const double* U, double* R, int parts, double* ms_vol, double* ms_surf) {
+// Face part of the residual of U: added to the volume part (have_vol) and written to
+// c->R, or, with `up`, fed to the SSP-RK stage update.  ms_flux / ms_test (nullable):
+// event-timed face-flux and face-test(+update) kernels; the one-pass owner-computes
+// kernel is accounted under ms_flux.
+void dispatch_faces(fvdg_ctx* c, const double* U, int parts, bool have_vol, const StageUpdate* up,
+                    double* ms_flux = nullptr, double* ms_test = nullptr) {
+    switch (c->order) {
+        case 0: faces_fast<0>(c, U, parts, have_vol, up, ms_flux, ms_test); break;
+        case 1: faces_fast<1>(c, U, parts, have_vol, up, ms_flux, ms_test); break;
+        case 2: faces_fast<2>(c, U, parts, have_vol, up, ms_flux, ms_test); break;
+        case 3: faces_fast<3>(c, U, parts, have_vol, up, ms_flux, ms_test); break;
+        case 4: faces_fast<4>(c, U, parts, have_vol, up, ms_flux, ms_test); break;
+        case 5: faces_fast<5>(c, U, parts, have_vol, up, ms_flux, ms_test); break;
+    }
+}
+
+// R = selected parts of the residual of U (device buffers; R is c->R)
+int launch_residual(fvdg_ctx* c, const double* U, int parts, double* ms_vol, double* ms_surf) {
     const SchemeTables& T = c->tab;
     const int bs = 256;
     if (c->simple) {
-        KernelArgs a = simple_args(c, U, R, parts);
+        KernelArgs a = simple_args(c, U, c->R.p, parts);
         Span sv(ms_vol, c->stream);
         if (parts & FVDG_PART_CELLS) {
             hipLaunchKernelGGL(k_vol_flux, dim3(nblocks(c->E * T.Q, bs)), dim3(bs), 0, c->stream, a);
@@ -354,11 +446,12 @@ int launch_residual(fvdg_ctx* c, const double* U, double* R, int parts, double* 
         hipLaunchKernelGGL(k_face_test, dim3(nblocks(c->E * T.N, bs)), dim3(bs), 0, c->stream, a);
         ss.stop();
     } else {
+        const bool vol = (parts & FVDG_PART_CELLS) != 0;
         Span sv(ms_vol, c->stream);
-        if (parts & FVDG_PART_CELLS) dispatch_vol(c, U, R);
+        if (vol) dispatch_vol(c, U);
         sv.stop();
         Span ss(ms_surf, c->stream);
-        dispatch_surf(c, surf_args(c, U, R, parts), false);
+        dispatch_faces(c, U, parts, vol, nullptr);
         ss.stop();
     }
     HIP_TRY(hipGetLastError());
@@ -378,7 +471,7 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
     double* dst[3] = {c->Ua.p, c->Ub.p, c->U.p};
     {
         if (c->simple) {
-            int rc = launch_residual(c, src[s], c->R.p, FVDG_PART_ALL, ms ? &ms[1] : nullptr, ms ? &ms[2] : nullptr);
+            int rc = launch_residual(c, src[s], FVDG_PART_ALL, ms ? &ms[1] : nullptr, ms ? &ms[2] : nullptr);
             if (rc) return rc;
             Span su(ms ? &ms[3] : nullptr, c->stream);
             const int64_t n = c->E * 5 * c->tab.N;
@@ -387,15 +480,10 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
             su.stop();
         } else {
             Span sv(ms ? &ms[1] : nullptr, c->stream);
-            dispatch_vol(c, src[s], c->R.p);
+            dispatch_vol(c, src[s]);
             sv.stop();
-            SurfArgs a = surf_args(c, src[s], c->R.p, FVDG_PART_ALL);
-            a.U0 = c->U.p;
-            a.out = dst[s];
-            a.a0 = A[s];
-            a.b0 = B[s];
-            a.dt = dt;
-            dispatch_surf(c, a, true, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
+            const StageUpdate up{c->U.p, dst[s], A[s], B[s], dt};
+            dispatch_faces(c, src[s], FVDG_PART_ALL, true, &up, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
         }
         HIP_TRY(hipGetLastError());
     }
@@ -447,7 +535,8 @@ int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precisio
     if (order < 0 || order > kMaxOrder) return fail(FVDG_ERR_UNSUPPORTED, "order must be 0..5");
     if (flux != FVDG_FLUX_LF && flux != FVDG_FLUX_ROE) return fail(FVDG_ERR_ARG, "unknown flux");
     if (!(gamma > 1.0)) return fail(FVDG_ERR_ARG, "gamma must exceed 1");
-    if (precision != FVDG_PREC_FP64) return fail(FVDG_ERR_UNSUPPORTED, "only FVDG_PREC_FP64 is available");
+    if (precision != FVDG_PREC_FP64 && precision != FVDG_PREC_FP32)
+        return fail(FVDG_ERR_UNSUPPORTED, "precision must be FVDG_PREC_FP64 or FVDG_PREC_FP32");
     HIP_TRY(hipSetDevice(c->device));
     const bool reorder = c->order != order;
     c->flux = flux;
@@ -459,6 +548,9 @@ int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precisio
         int rc = upload_tables(c);
         if (rc) return rc;
         rc = alloc_state(c);
+        if (rc) return rc;
+    } else {
+        const int rc = precision_buffers(c);  // keeps an uploaded state
         if (rc) return rc;
     }
     return FVDG_OK;
@@ -604,7 +696,7 @@ int fvdg_residual(fvdg_ctx* c, double t, int parts, double* Rh) {
     if (rc) return rc;
     if (parts & ~FVDG_PART_ALL) return fail(FVDG_ERR_ARG, "unknown residual part bits");
     HIP_TRY(hipSetDevice(c->device));
-    rc = launch_residual(c, c->U.p, c->R.p, parts, nullptr, nullptr);
+    rc = launch_residual(c, c->U.p, parts, nullptr, nullptr);
     if (rc) return rc;
     if (Rh)
         HIP_TRY(hipMemcpyAsync(Rh, c->R.p, (size_t)c->E * 5 * c->tab.N * sizeof(double), hipMemcpyDeviceToHost,
