@@ -115,11 +115,8 @@ SchemeTables build_tables(int order) {
     return T;
 }
 
-double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& PhiC, std::vector<uint8_t>& canon,
-                                   std::vector<int32_t>& perm) {
-    const int N = T.N, QF = T.Qf;
-    // canonical orders: for each vertex subset {x<y<z}: (x,y,z) then (x,z,y)
-    int canon_combo[8];
+// canonical orders: for each vertex subset {x<y<z} in lexicographic order: (x,y,z) then (x,z,y)
+static void canonical_combos(int canon_combo[8]) {
     int n = 0;
     for (int x = 0; x < 4; ++x)
         for (int y = x + 1; y < 4; ++y)
@@ -127,6 +124,27 @@ double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& P
                 canon_combo[n++] = combo_index(x, y, z);
                 canon_combo[n++] = combo_index(x, z, y);
             }
+}
+
+int canonical_of_combo(int combo) {
+    int canon_combo[8];
+    canonical_combos(canon_combo);
+    int a[3];
+    combo_vertices(combo, a);
+    int m = 0;
+    for (int i = 1; i < 3; ++i)
+        if (a[i] < a[m]) m = i;
+    const int cc = combo_index(a[m], a[(m + 1) % 3], a[(m + 2) % 3]);
+    for (int i = 0; i < 8; ++i)
+        if (canon_combo[i] == cc) return i * 3 + m;
+    return -1;
+}
+
+double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& PhiC, std::vector<uint8_t>& canon,
+                                   std::vector<int32_t>& perm) {
+    const int N = T.N, QF = T.Qf;
+    int canon_combo[8];
+    canonical_combos(canon_combo);
     PhiC.assign(8 * QF * N, 0.0);
     for (int c = 0; c < 8; ++c)
         for (int g = 0; g < QF; ++g)
@@ -152,17 +170,8 @@ double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& P
     canon.assign(24, 0);
     double dev = 0.0;
     for (int c = 0; c < 24; ++c) {
-        int a[3];
-        combo_vertices(c, a);
-        int m = 0;
-        for (int i = 1; i < 3; ++i)
-            if (a[i] < a[m]) m = i;
-        const int r0 = a[m], r1 = a[(m + 1) % 3], r2 = a[(m + 2) % 3];
-        const int cc = combo_index(r0, r1, r2);
-        int ci = -1;
-        for (int i = 0; i < 8; ++i)
-            if (canon_combo[i] == cc) ci = i;
-        canon[c] = (uint8_t)(ci * 3 + m);
+        const int cm = canonical_of_combo(c), ci = cm / 3, m = cm % 3;
+        canon[c] = (uint8_t)cm;
         for (int g = 0; g < QF; ++g)
             for (int k = 0; k < N; ++k)
                 dev = std::max(dev, std::abs(T.Phi[((size_t)c * QF + g) * N + k] - PhiC[(ci * QF + perm[m * QF + g]) * N + k]));

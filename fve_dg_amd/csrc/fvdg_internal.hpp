@@ -43,6 +43,10 @@ SchemeTables build_tables(int order);
 // vertex orders (starting at the smallest reference vertex), per combo its
 // canonical index*3 + rotation, and the 3 point permutations of the rule.
 // Returns the largest deviation |Phi[c][g][k] - PhiC[canon][perm[rot][g]][k]|.
+// canonical index * 3 + rotation of a combo (canonical index ci: vertex subset ci/2 in
+// lexicographic order {0,1,2},{0,1,3},{0,2,3},{1,2,3}, i.e. the face opposite vertex
+// 3 - ci/2; ci%2 = 0 ascending order, 1 the other rotation class)
+int canonical_of_combo(int combo);
 double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& PhiC, std::vector<uint8_t>& canon,
                                    std::vector<int32_t>& perm);
 

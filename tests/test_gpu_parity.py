@@ -194,3 +194,29 @@ def test_reference_meshes_divergence_identity(solver, oracle, mesh_name, order, 
     Rx = divergence_rhs(m, order, og["det"], st.div_flux)
     scale = np.abs(Rx).max() if order == 1 else face_scale(oracle, m, og, U)
     assert np.abs(Rg - Rx).max() <= (1e-12 if order == 1 else 1e-9) * scale
+
+
+@pytest.mark.parametrize("order,flux,n", [(1, "lf", 6), (3, "roe", 8), (5, "roe", 4)])
+def test_face_paths_agree(oracle, order, flux, n, monkeypatch):
+    """The two-pass face kernels (default) and the one-pass owner-computes kernel
+    (FVDG_FACES=owner, chosen when a context is created) give the same residual and step."""
+    print(f"[case] face paths order={order} flux={flux} n={n}", flush=True)
+    mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=5)
+    U = perturbed_vortex(mesh, order, seed=3)
+    out = {}
+    for path in ("default", "owner"):
+        if path == "default":
+            monkeypatch.delenv("FVDG_FACES", raising=False)
+        else:
+            monkeypatch.setenv("FVDG_FACES", path)
+        s = F.Solver(0)
+        s.set_scheme(order, flux=flux).set_mesh(mesh)
+        s.upload(U)
+        R = s.residual()
+        s.ssprk3(0.01, 2)
+        out[path] = (R, s.download())
+        s.close()
+    Ro = oracle.residual(mesh.export(), order, U, flux=flux)
+    for path, (R, U2) in out.items():
+        assert rel_err(R, Ro) < TOL, path
+        assert rel_err(U2, out["default"][1]) < 1e-13, path
