@@ -85,6 +85,18 @@ def lib():
         _sig(L, "fvdg_max_wave_speed", [_vp, _d])
         _sig(L, "fvdg_conservation_sums", [_vp, _d])
         _sig(L, "fvdg_time_stages", [_vp, C.c_double, C.c_int, _d])
+        _sig(L, "fvdg_ensemble_create", [C.POINTER(_vp), C.c_int, C.c_int, C.POINTER(_vp), C.POINTER(C.c_int),
+                                         C.c_int, C.c_double, C.c_int])
+        _sig(L, "fvdg_ensemble_destroy", [_vp], None)
+        _sig(L, "fvdg_ensemble_num_cells", [_vp, C.c_int, _i64])
+        _sig(L, "fvdg_ensemble_num_groups", [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)])
+        _sig(L, "fvdg_ensemble_upload", [_vp, C.c_int, _d])
+        _sig(L, "fvdg_ensemble_download", [_vp, C.c_int, _d])
+        _sig(L, "fvdg_ensemble_max_wave_speeds", [_vp, _d])
+        _sig(L, "fvdg_ensemble_set_dt", [_vp, _d])
+        _sig(L, "fvdg_ensemble_step", [_vp, C.c_int, C.c_int])
+        _sig(L, "fvdg_ensemble_time_steps", [_vp, C.c_int, C.c_int, _d])
+        _sig(L, "fvdg_ensemble_autotune", [_vp, C.c_int, C.POINTER(C.c_int)])
         _lib = L
     return _lib
 
@@ -312,3 +324,76 @@ class Solver:
         ms = np.zeros(4)
         _check(lib().fvdg_time_stages(self._h, dt, n_stages, _p(ms)))
         return ms
+
+
+class Ensemble:
+    """Many independent problems (meshes of uneven size, orders 0..5) stepped together on
+    one GPU: per order one union mesh, per problem its own time step, optional HIP graph."""
+
+    def __init__(self, meshes, orders, flux="lf", gamma=1.4, precision="fp64", device=0):
+        self.n = len(meshes)
+        self.orders = [int(o) for o in orders]
+        self._meshes = list(meshes)  # keep the host meshes alive for the caller's convenience
+        arr = (_vp * self.n)(*[m._h for m in meshes])
+        ords = (C.c_int * self.n)(*self.orders)
+        h = _vp()
+        _check(lib().fvdg_ensemble_create(C.byref(h), device, self.n, arr, ords, FLUX[flux], gamma,
+                                          PRECISION[precision]))
+        self._h = h
+        self.n_cells = []
+        for p in range(self.n):
+            nc = C.c_int64()
+            _check(lib().fvdg_ensemble_num_cells(self._h, p, C.byref(nc)))
+            self.n_cells.append(nc.value)
+
+    def groups(self):
+        n = C.c_int()
+        orders = (C.c_int * 8)()
+        _check(lib().fvdg_ensemble_num_groups(self._h, C.byref(n), orders))
+        return list(orders[: n.value])
+
+    def upload(self, p, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != (self.n_cells[p], 5 * num_basis(self.orders[p])):
+            raise FvdgError(FVDG_ERR_ARG, f"state of problem {p} has shape {U.shape}")
+        _check(lib().fvdg_ensemble_upload(self._h, p, _p(U)))
+
+    def download(self, p):
+        U = np.zeros((self.n_cells[p], 5 * num_basis(self.orders[p])))
+        _check(lib().fvdg_ensemble_download(self._h, p, _p(U)))
+        return U
+
+    def max_wave_speeds(self):
+        out = np.zeros(self.n)
+        _check(lib().fvdg_ensemble_max_wave_speeds(self._h, _p(out)))
+        return out
+
+    def set_dt(self, dt):
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.n,)))
+        _check(lib().fvdg_ensemble_set_dt(self._h, _p(dt)))
+
+    def step(self, n_steps=1, graph=True):
+        _check(lib().fvdg_ensemble_step(self._h, n_steps, int(graph)))
+
+    def time_steps(self, n_steps, graph=True):
+        ms = C.c_double()
+        _check(lib().fvdg_ensemble_time_steps(self._h, n_steps, int(graph), C.byref(ms)))
+        return ms.value
+
+    def autotune(self, trial_steps=2):
+        ng = len(self.groups())
+        ch = (C.c_int * (2 * ng))()
+        _check(lib().fvdg_ensemble_autotune(self._h, trial_steps, ch))
+        return {o: {"owner_faces": bool(ch[2 * i]), "blocks_per_cu": ch[2 * i + 1]}
+                for i, o in enumerate(self.groups())}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fvdg_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

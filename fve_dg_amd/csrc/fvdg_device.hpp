@@ -1,0 +1,107 @@
+// Device-side internals shared by the solver (fvdg_solver.hip) and the ensemble
+// (fvdg_ensemble.hip) translation units of libfvdg.so: the solver context and the
+// stage driver.  Not installed; the public ABI is include/fvdg.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fvdg_internal.hpp"
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return fail(FVDG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+namespace fvdg {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(&p, count * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& v, hipStream_t s) {
+        hipError_t e = alloc(v.size());
+        if (e != hipSuccess || v.empty()) return e;
+        e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipStreamSynchronize(s);
+    }
+};
+
+}  // namespace fvdg
+
+using fvdg::DevBuf;
+using fvdg::SchemeTables;
+
+struct fvdg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = true;  // false: a stream lent by an ensemble
+    int grid_mult = 8;        // persistent face kernels: blocks per compute unit (tuned by ensembles)
+    // scheme
+    int order = -1, flux = FVDG_FLUX_LF, precision = FVDG_PREC_FP64;
+    double gamma = 1.4;
+    SchemeTables tab;
+    // mesh
+    int64_t E = 0, Epad = 0, F = 0, n_dir = 0;
+    bool have_mesh = false, have_state = false;
+    std::vector<int64_t> dir_faces;       // Dirichlet face ids, host copy
+    std::vector<double> dir_xyz;          // their face quadrature points
+    std::vector<double> cell_det_host;    // for the mass diagonal
+    double h_min = 0;                     // min over cells of 3 * volume / (sum of face areas)
+    // device geometry
+    DevBuf<double> cgeo;        // E*10: A^{-1} (9), |det A|
+    DevBuf<int32_t> cfaces;     // E*4: face*2 + side
+    DevBuf<int32_t> fcell;      // F*2
+    DevBuf<uint8_t> fcombo;     // F*2
+    DevBuf<double> fgeo;        // F*4: normal (3), 2*area
+    DevBuf<uint8_t> fbc;        // F: 255 interior, else BoundaryType
+    DevBuf<int32_t> fdir;       // F: Dirichlet slot or -1
+    DevBuf<double> UD;          // n_dir*Qf*5
+    // device tables
+    DevBuf<double> tV, tG, tWq, tPhi, tWf, tMhat;
+    DevBuf<double> tVA, tGA, tWqa, tPhiT;  // zero-padded MFMA operand tables, transposed face table
+    DevBuf<double> tPhiC;                 // canonical face tables (8 vertex orders)
+    DevBuf<float> tVA32, tGA32, tPhiC32;  // FP32 copies for the mixed-precision kernels
+    DevBuf<double> tGs, tIC;              // mixed: sum_q w_q grad phi_j (N*3), sum_g w_g PhiC (8*N)
+    DevBuf<uint8_t> tCanon;               // per combo: canonical index * 3 + rotation
+    DevBuf<int32_t> tPerm;                // face-rule point permutations of the 3 rotations
+    int n_cu = 256;
+    // device state: U (state), Ua/Ub (stage ping-pong), R (residual)
+    DevBuf<double> U, Ua, Ub, R;
+    DevBuf<double> Fq, Fh;           // Fq: simple-kernel scratch E*Q*15; Fh: face fluxes F*Qf*5
+    DevBuf<float> Fh32;              // mixed precision: face flux differences F*Qf*5
+    DevBuf<double> Fref;             // mixed precision: per-face reference flux F*5
+    bool simple = false;             // FVDG_KERNELS=simple
+    bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
+    DevBuf<double> red;              // reduction partials
+    DevBuf<double> edt;              // per-element time step (ensembles); empty: the call's dt
+    int64_t n_red = 0;
+
+    ~fvdg_ctx() {
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace fvdg {
+// one SSP-RK3 step of the context's device state (stream-ordered, not synchronised);
+// ms (nullable): per-kernel event times, accumulated
+int ssprk3_once(fvdg_ctx* c, double dt, double* ms);
+// per face: max over its points of the traces' |u| + c (host copy of F values)
+int face_wave_speeds(fvdg_ctx* c, std::vector<double>& out);
+}  // namespace fvdg

@@ -242,6 +242,38 @@ static void parallel_for(int64_t n, Fn fn) {
 using namespace fvdg;
 
 // ================================================================ C ABI (host part)
+// Disjoint union of meshes (the problems of an ensemble): points, cells and faces of part i
+// follow those of parts 0..i-1, every index shifted; boundary types are kept.
+fvdg_mesh* fvdg::union_mesh(const std::vector<const fvdg_mesh*>& parts, std::vector<int64_t>& cell_off,
+                            std::vector<int64_t>& face_off) {
+    std::vector<vector3f> pts;
+    std::vector<uInt> cn, fn, fc, fr;
+    std::vector<BoundaryType> bc;
+    cell_off.assign(1, 0);
+    face_off.assign(1, 0);
+    for (const fvdg_mesh* m : parts) {
+        const auto& cm = m->cm;
+        const uInt p0 = pts.size(), c0 = cell_off.back();
+        pts.insert(pts.end(), cm.m_points.begin(), cm.m_points.end());
+        for (const auto& c : cm.m_cells)
+            for (int a = 0; a < 4; ++a) cn.push_back(c.m_nodes[a] + p0);
+        for (const auto& F : cm.m_faces) {
+            for (int k = 0; k < 3; ++k) {
+                fn.push_back(F.m_nodes[k] + p0);
+                fr.push_back(F.m_nodes_r[k] + p0);
+            }
+            fc.push_back(F.m_neighbor_cells[0] + c0);
+            fc.push_back(F.is_boundary() ? uInt(-1) : F.m_neighbor_cells[1] + c0);
+        }
+        bc.insert(bc.end(), cm.m_boundaryTypes.begin(), cm.m_boundaryTypes.end());
+        cell_off.push_back(c0 + cm.m_cells.size());
+        face_off.push_back(face_off.back() + cm.m_faces.size());
+    }
+    auto* u = new fvdg_mesh{ComputingMesh::from_arrays(pts, cn, fn, fc, fr)};
+    u->cm.m_boundaryTypes = bc;
+    return u;
+}
+
 extern "C" {
 
 const char* fvdg_version(void) { return "fvdg 0.1 (gfx950)"; }

@@ -54,6 +54,10 @@ double build_canonical_face_tables(const SchemeTables& T, std::vector<double>& P
 // the reference's operation order nc0*(1-u-v) + nc1*u + nc2*v.
 vector3f face_point_to_cell(int combo, double u, double v);
 
+// Disjoint union of meshes; cell_off / face_off get the n+1 prefix offsets of the parts.
+fvdg_mesh* union_mesh(const std::vector<const fvdg_mesh*>& parts, std::vector<int64_t>& cell_off,
+                      std::vector<int64_t>& face_off);
+
 // Analytic fields (host): conserved state at a physical point.
 bool analytic_state(int field, const double* params, int n_params, double t, const vector3f& x, double U[5]);
 

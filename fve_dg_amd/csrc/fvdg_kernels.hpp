@@ -307,7 +307,8 @@ __global__ void k_face_test(KernelArgs a) {
 // out = a0*U0 + b0*(Us + dt * (-R / M)),  M = |det A_e| * mass_hat[k]
 __global__ void k_rk_update(const double* __restrict__ U0, const double* __restrict__ Us,
                             const double* __restrict__ R, double* __restrict__ out, const double* __restrict__ cgeo,
-                            const double* __restrict__ mhat, int64_t E, int N, double a0, double b0, double dt) {
+                            const double* __restrict__ mhat, int64_t E, int N, double a0, double b0, double dt,
+                            const double* __restrict__ edt) {
     const int NV = 5 * N;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E * NV) return;
@@ -315,7 +316,7 @@ __global__ void k_rk_update(const double* __restrict__ U0, const double* __restr
     const int kv = (int)(i - e * NV);
     const double M = cgeo[e * 10 + 9] * mhat[kv / 5];
     const double L = -R[i] / M;
-    const double s = Us[i] + dt * L;
+    const double s = Us[i] + (edt ? edt[e] : dt) * L;
     out[i] = (a0 == 0.0) ? b0 * s : a0 * U0[i] + b0 * s;
 }
 
@@ -342,6 +343,23 @@ __global__ void k_max_wave_speed(KernelArgs a, double* __restrict__ part) {
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// per face: max over its points of max(|u|+c) of the traces (one thread per face)
+__global__ void k_face_wave_speed(KernelArgs a, double* __restrict__ out) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.F) return;
+    double m = 0.0;
+    for (int g = 0; g < a.Qf; ++g) {
+        double U[5];
+        face_trace(a, a.fcell[2 * f], a.fcombo[2 * f], g, U);
+        m = fmax(m, euler_wave_speed(U, a.gamma));
+        if (a.fcell[2 * f + 1] >= 0) {
+            face_trace(a, a.fcell[2 * f + 1], a.fcombo[2 * f + 1], g, U);
+            m = fmax(m, euler_wave_speed(U, a.gamma));
+        }
+    }
+    out[f] = m;
 }
 
 // integral of each conserved variable: |det A_e| * c_{e,0,v} * (1/6), phi_0 = 1

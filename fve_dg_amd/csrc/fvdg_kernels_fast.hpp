@@ -263,11 +263,13 @@ struct SurfArgs {
     const double* Phi;   // 24*QF*N  (c, g, k)
     const double* PhiT;  // 24*N*QF  (c, k, g)
     const double* wf;
-    // fused SSP-RK stage: out = a0*U0 + b0*(U + dt*(-R/M)), M = |det| * mhat[k]
+    // fused SSP-RK stage: out = a0*U0 + b0*(U + dt_e*(-R/M)), M = |det| * mhat[k],
+    // dt_e = edt[e] when edt is set (problems of an ensemble), else dt
     const double* U0;
     double* out;
     const double* mhat;
     double a0, b0, dt;
+    const double* edt;
 };
 
 // ------------------------------------------------------------------ surface, persistent
@@ -479,7 +481,7 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     const double Rt = a.R[i0 + v] + r[it][v];
-                    const double sv = a.U[i0 + v] + a.dt * (-Rt / M);
+                    const double sv = a.U[i0 + v] + (a.edt ? a.edt[e] : a.dt) * (-Rt / M);
                     a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
                 }
             } else {
@@ -697,11 +699,13 @@ struct FaceTestArgs {
     const int32_t* cfaces;
     const int32_t* fcell;
     const uint8_t* fcombo;
-    // fused SSP-RK stage: out = a0*U0 + b0*(U + dt*(-R/M)), M = |det| * mhat[k]
+    // fused SSP-RK stage: out = a0*U0 + b0*(U + dt_e*(-R/M)), M = |det| * mhat[k],
+    // dt_e = edt[e] when edt is set, else dt
     const double* U0;
     double* out;
     const double* mhat;
     double a0, b0, dt;
+    const double* edt;
     const T* Fh;
     const double* Fref;   // F*5 (FP32 path only)
     const double* IC;     // 8*N: sum_g w_g PhiC[c][g][j] (FP32 path only)
@@ -811,7 +815,7 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs<T> a) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     const double Rt = a.Rv[i0 + v] + (rc[v] + (double)r[v]);
-                    const double sv = a.U[i0 + v] + a.dt * (-Rt / M);
+                    const double sv = a.U[i0 + v] + (a.edt ? a.edt[e] : a.dt) * (-Rt / M);
                     a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
                 }
             } else {

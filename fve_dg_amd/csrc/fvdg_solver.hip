@@ -22,94 +22,12 @@
 #include <cstring>
 #include <memory>
 
+#include "fvdg_device.hpp"
 #include "fvdg_internal.hpp"
 #include "fvdg_kernels.hpp"
 #include "fvdg_kernels_fast.hpp"
 
 using namespace fvdg;
-
-#define HIP_TRY(expr)                                                                         \
-    do {                                                                                      \
-        hipError_t _e = (expr);                                                               \
-        if (_e != hipSuccess)                                                                 \
-            return fail(FVDG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
-    } while (0)
-
-namespace {
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    hipError_t alloc(size_t count) {
-        release();
-        n = count;
-        if (count == 0) return hipSuccess;
-        return hipMalloc(&p, count * sizeof(T));
-    }
-    hipError_t upload(const std::vector<T>& v, hipStream_t s) {
-        hipError_t e = alloc(v.size());
-        if (e != hipSuccess || v.empty()) return e;
-        e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return e;
-        return hipStreamSynchronize(s);
-    }
-};
-
-}  // namespace
-
-struct fvdg_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    // scheme
-    int order = -1, flux = FVDG_FLUX_LF, precision = FVDG_PREC_FP64;
-    double gamma = 1.4;
-    SchemeTables tab;
-    // mesh
-    int64_t E = 0, Epad = 0, F = 0, n_dir = 0;
-    bool have_mesh = false, have_state = false;
-    std::vector<int64_t> dir_faces;       // Dirichlet face ids, host copy
-    std::vector<double> dir_xyz;          // their face quadrature points
-    std::vector<double> cell_det_host;    // for the mass diagonal
-    double h_min = 0;                     // min over cells of 3 * volume / (sum of face areas)
-    // device geometry
-    DevBuf<double> cgeo;        // E*10: A^{-1} (9), |det A|
-    DevBuf<int32_t> cfaces;     // E*4: face*2 + side
-    DevBuf<int32_t> fcell;      // F*2
-    DevBuf<uint8_t> fcombo;     // F*2
-    DevBuf<double> fgeo;        // F*4: normal (3), 2*area
-    DevBuf<uint8_t> fbc;        // F: 255 interior, else BoundaryType
-    DevBuf<int32_t> fdir;       // F: Dirichlet slot or -1
-    DevBuf<double> UD;          // n_dir*Qf*5
-    // device tables
-    DevBuf<double> tV, tG, tWq, tPhi, tWf, tMhat;
-    DevBuf<double> tVA, tGA, tWqa, tPhiT;  // zero-padded MFMA operand tables, transposed face table
-    DevBuf<double> tPhiC;                 // canonical face tables (8 vertex orders)
-    DevBuf<float> tVA32, tGA32, tPhiC32;  // FP32 copies for the mixed-precision kernels
-    DevBuf<double> tGs, tIC;              // mixed: sum_q w_q grad phi_j (N*3), sum_g w_g PhiC (8*N)
-    DevBuf<uint8_t> tCanon;               // per combo: canonical index * 3 + rotation
-    DevBuf<int32_t> tPerm;                // face-rule point permutations of the 3 rotations
-    int n_cu = 256;
-    // device state: U (state), Ua/Ub (stage ping-pong), R (residual)
-    DevBuf<double> U, Ua, Ub, R;
-    DevBuf<double> Fq, Fh;           // Fq: simple-kernel scratch E*Q*15; Fh: face fluxes F*Qf*5
-    DevBuf<float> Fh32;              // mixed precision: face flux differences F*Qf*5
-    DevBuf<double> Fref;             // mixed precision: per-face reference flux F*5
-    bool simple = false;             // FVDG_KERNELS=simple
-    bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
-    DevBuf<double> red;              // reduction partials
-    int64_t n_red = 0;
-
-    ~fvdg_ctx() {
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 
@@ -267,6 +185,7 @@ SurfArgs surf_args(fvdg_ctx* c, const double* U, double* R, int parts) {
     a.out = nullptr;
     a.mhat = c->tMhat.p;
     a.a0 = a.b0 = a.dt = 0.0;
+    a.edt = c->edt.n ? c->edt.p : nullptr;
     return a;
 }
 
@@ -334,7 +253,7 @@ void launch_surf_fast(fvdg_ctx* c, const SurfArgs& a, bool fuse) {
     constexpr int TE = SurfLdsShape<P>::TE, NT = SurfLdsShape<P>::THREADS;
     SurfLdsArgs A{a, c->tPhiC.p, c->tCanon.p, c->tPerm.p, (int64_t)nblocks(c->E, TE)};
     // persistent grid: a few blocks per compute unit, each walking tiles
-    const unsigned grid = (unsigned)std::min<int64_t>(A.ntiles, (int64_t)c->n_cu * 4);
+    const unsigned grid = (unsigned)std::min<int64_t>(A.ntiles, (int64_t)c->n_cu * std::max(1, c->grid_mult / 2));
     if (fuse)
         hipLaunchKernelGGL((k_surf_lds<P, true>), dim3(grid), dim3(NT), 0, c->stream, A);
     else
@@ -348,7 +267,7 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
                       c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0};
     constexpr int FB = FaceFluxShape<P>::FB;
     f.ntiles = (c->F + FB - 1) / FB;
-    const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * 8);
+    const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * c->grid_mult);
     Span s1(ms_flux, c->stream);
     hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P>::THREADS), 0, c->stream, f);
     s1.stop();
@@ -364,6 +283,7 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.fcell = c->fcell.p;
     t.fcombo = c->fcombo.p;
     t.mhat = c->tMhat.p;
+    t.edt = c->edt.n ? c->edt.p : nullptr;
     if (up) {
         t.U0 = up->U0;
         t.out = up->out;
@@ -378,7 +298,7 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.canon = c->tCanon.p;
     t.perm = c->tPerm.p;
     t.ntiles = (c->E + TE - 1) / TE;
-    const unsigned g2 = (unsigned)std::min<int64_t>(t.ntiles, (int64_t)c->n_cu * 8);
+    const unsigned g2 = (unsigned)std::min<int64_t>(t.ntiles, (int64_t)c->n_cu * c->grid_mult);
     Span s2(ms_test, c->stream);
     if (up)
         hipLaunchKernelGGL((k_face_test_lds<P, true, T>), dim3(g2), dim3(FaceTestShape<P>::THREADS), 0, c->stream, t);
@@ -406,6 +326,7 @@ void faces_fast(fvdg_ctx* c, const double* U, int parts, bool have_vol, const St
             a.b0 = up->b0;
             a.dt = up->dt;
         }
+        a.edt = c->edt.n ? c->edt.p : nullptr;
         Span so(ms_flux, c->stream);
         launch_surf_fast<P>(c, a, up != nullptr);
         so.stop();
@@ -476,7 +397,8 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
             Span su(ms ? &ms[3] : nullptr, c->stream);
             const int64_t n = c->E * 5 * c->tab.N;
             hipLaunchKernelGGL(k_rk_update, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, c->U.p, src[s], c->R.p,
-                               dst[s], c->cgeo.p, c->tMhat.p, c->E, c->tab.N, A[s], B[s], dt);
+                               dst[s], c->cgeo.p, c->tMhat.p, c->E, c->tab.N, A[s], B[s], dt,
+                               c->edt.n ? c->edt.p : nullptr);
             su.stop();
         } else {
             Span sv(ms ? &ms[1] : nullptr, c->stream);
@@ -490,7 +412,10 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
     return FVDG_OK;
 }
 
-int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
+
+}  // namespace
+
+int fvdg::ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
     for (int s = 0; s < 3; ++s) {
         const int rc = ssprk3_stage(c, dt, s, ms);
         if (rc) return rc;
@@ -498,7 +423,20 @@ int ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
     return FVDG_OK;
 }
 
-}  // namespace
+int fvdg::face_wave_speeds(fvdg_ctx* c, std::vector<double>& out) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    KernelArgs a = simple_args(c, c->U.p, c->R.p, FVDG_PART_ALL);
+    DevBuf<double> d;
+    HIP_TRY(d.alloc(c->F));
+    hipLaunchKernelGGL(k_face_wave_speed, dim3(nblocks(c->F, 256)), dim3(256), 0, c->stream, a, d.p);
+    HIP_TRY(hipGetLastError());
+    out.resize(c->F);
+    HIP_TRY(hipMemcpyAsync(out.data(), d.p, c->F * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
 
 // ================================================================ C ABI (device part)
 extern "C" {

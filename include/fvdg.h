@@ -157,6 +157,33 @@ int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);
  * (per-stage averages). */
 int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms);
 
+/* ---------------- ensembles: many independent problems of uneven size and order ----------------
+ * (no counterpart in the reference, whose drivers run one problem at a time.)  Problems of one
+ * order are stepped as one disjoint-union mesh with per-problem time steps; one SSP-RK3 step of
+ * the whole ensemble can be captured into a HIP graph and replayed. */
+typedef struct fvdg_ensemble fvdg_ensemble;
+/* n problems: meshes[i] (caller keeps ownership; copied) at order orders[i] (0..5) */
+int fvdg_ensemble_create(fvdg_ensemble** out, int device, int n, const fvdg_mesh* const* meshes, const int* orders,
+                         int flux, double gamma, int precision);
+void fvdg_ensemble_destroy(fvdg_ensemble* e);
+int fvdg_ensemble_num_cells(fvdg_ensemble* e, int problem, int64_t* n_cells);
+int fvdg_ensemble_num_groups(fvdg_ensemble* e, int* n_groups, int* orders);
+/* state of one problem, LongVector layout (n_cells x 5N) */
+int fvdg_ensemble_upload(fvdg_ensemble* e, int problem, const double* U);
+int fvdg_ensemble_download(fvdg_ensemble* e, int problem, double* U);
+/* per problem: max over face points of |u| + c of the device state (out[n]) */
+int fvdg_ensemble_max_wave_speeds(fvdg_ensemble* e, double* out);
+/* per-problem time steps dt[n] */
+int fvdg_ensemble_set_dt(fvdg_ensemble* e, const double* dt);
+/* n_steps SSP-RK3 steps of every problem; use_graph: replay a captured HIP graph of one step */
+int fvdg_ensemble_step(fvdg_ensemble* e, int n_steps, int use_graph);
+/* average ms per ensemble step over n_steps (after one untimed step) */
+int fvdg_ensemble_time_steps(fvdg_ensemble* e, int n_steps, int use_graph, double* ms_per_step);
+/* per order group, time the face-kernel variant (two-pass / owner-computes, FP64 only) and
+ * the persistent blocks per compute unit (2, 4, 8, 16) over trial_steps steps on the
+ * ensemble's own state, keep the fastest (state restored); choices[2g] = owner, [2g+1] = blocks */
+int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices);
+
 #ifdef __cplusplus
 }
 #endif
