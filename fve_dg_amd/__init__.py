@@ -67,6 +67,13 @@ def lib():
         _sig(L, "fvdg_mesh_get_boundary_types", [_vp, _u8])
         _sig(L, "fvdg_mesh_tag_boundaries_pseudo3dz", [_vp])
         _sig(L, "fvdg_mesh_destroy", [_vp], None)
+        _sig(L, "fvdg_mesh_read_fluent", [C.POINTER(_vp), C.c_char_p])
+        _sig(L, "fvdg_mesh_face_zones", [_vp, C.POINTER(C.c_int32)])
+        _sig(L, "fvdg_mesh_zones", [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p,
+                                    C.c_int])
+        _sig(L, "fvdg_mesh_set_zone_boundary_type", [_vp, C.c_int32, C.c_uint8])
+        _sig(L, "fvdg_solution_write", [_vp, C.c_int, _d, C.c_int, _d, C.c_int, C.c_double, C.c_char_p])
+        _sig(L, "fvdg_solution_read", [_vp, C.c_int, C.c_char_p, _d])
         _sig(L, "fvdg_project_analytic", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d])
         _sig(L, "fvdg_analytic_error", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d, _d])
         _sig(L, "fvdg_create", [C.POINTER(_vp), C.c_int])
@@ -173,6 +180,44 @@ class Mesh:
                                       _p(fn, C.c_int64), _p(fc, C.c_int64),
                                       None if fr is None else _p(fr, C.c_int64)))
         return cls(h)
+
+    @classmethod
+    def read_fluent(cls, path):
+        """Fluent ASCII .msh file -> conforming tetrahedral mesh (non-tet cells split around
+        their centroids); boundary faces typed from their zones."""
+        h = _vp()
+        _check(lib().fvdg_mesh_read_fluent(C.byref(h), str(path).encode()))
+        return cls(h)
+
+    def zones(self):
+        """{zone id: (Fluent boundary-condition code, name)} of a mesh read from a file"""
+        n = C.c_int()
+        _check(lib().fvdg_mesh_zones(self._h, C.byref(n), None, None, None, 0))
+        ids = (C.c_int32 * n.value)()
+        bct = (C.c_int32 * n.value)()
+        names = C.create_string_buffer(64 * max(n.value, 1))
+        _check(lib().fvdg_mesh_zones(self._h, C.byref(n), ids, bct, names, 64))
+        return {ids[i]: (bct[i], names.raw[64 * i: 64 * (i + 1)].split(b"\0")[0].decode()) for i in range(n.value)}
+
+    def face_zones(self):
+        z = np.zeros(self.sizes[2], dtype=np.int32)
+        _check(lib().fvdg_mesh_face_zones(self._h, z.ctypes.data_as(C.POINTER(C.c_int32))))
+        return z
+
+    def set_zone_boundary_type(self, zone, bc):
+        _check(lib().fvdg_mesh_set_zone_boundary_type(self._h, int(zone), int(bc)))
+
+    def write_solution(self, path, order, U, field=None, params=(), t=0.0):
+        """the reference's T_k_N_n.txt layout: x y z (U_h U_s) x 5 per cell and volume point"""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        _check(lib().fvdg_solution_write(self._h, order, _p(U), -1 if field is None else FIELD[field],
+                                         _p(prm) if prm.size else None, prm.size, t, str(path).encode()))
+
+    def read_solution(self, path, order):
+        U = np.zeros((self.n_cells, 5 * num_basis(order)))
+        _check(lib().fvdg_solution_read(self._h, order, str(path).encode(), _p(U)))
+        return U
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

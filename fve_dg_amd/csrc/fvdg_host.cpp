@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <thread>
 
 #include "Mesh/Mesh.h"
@@ -366,6 +369,67 @@ int fvdg_mesh_create(fvdg_mesh** out, int64_t n_points, const double* points, in
     }
 }
 
+int fvdg_mesh_read_fluent(fvdg_mesh** out, const char* path) {
+    if (!out || !path) return fail(FVDG_ERR_ARG, "null argument");
+    try {
+        FluentMesh fm;
+        fm.read(path);
+        std::vector<int32_t> fz;
+        GeneralMesh g = fm.to_tets(fz);
+        auto* m = new fvdg_mesh{ComputingMesh(g), fz, {}};
+        for (const auto& kv : fm.zones) m->zones.push_back(kv.second);
+        std::map<int, int> bc_of;
+        for (const auto& z : m->zones) bc_of[z.id] = z.bc_type;
+        for (size_t f = 0; f < m->cm.m_faces.size(); ++f)
+            if (m->cm.m_faces[f].is_boundary())
+                m->cm.m_boundaryTypes[f] = fz[f] >= 0 ? FluentMesh::boundary_type_of(bc_of[fz[f]]) : BoundaryType::Wall;
+        for (const auto& c : m->cm.m_cells)
+            if (!(c.m_volume > 0)) {
+                delete m;
+                return fail(FVDG_ERR_MESH, "degenerate tetrahedron after splitting");
+            }
+        *out = m;
+        return FVDG_OK;
+    } catch (const std::exception& e) {
+        return fail(FVDG_ERR_MESH, e.what());
+    }
+}
+
+int fvdg_mesh_face_zones(const fvdg_mesh* m, int32_t* zone) {
+    if (!m || !zone) return fail(FVDG_ERR_ARG, "null argument");
+    for (size_t f = 0; f < m->cm.m_faces.size(); ++f) zone[f] = f < m->face_zone.size() ? m->face_zone[f] : -1;
+    return FVDG_OK;
+}
+
+int fvdg_mesh_zones(const fvdg_mesh* m, int* n, int32_t* ids, int32_t* bc_types, char* names, int name_len) {
+    if (!m || !n) return fail(FVDG_ERR_ARG, "null argument");
+    *n = (int)m->zones.size();
+    for (size_t i = 0; i < m->zones.size(); ++i) {
+        if (ids) ids[i] = m->zones[i].id;
+        if (bc_types) bc_types[i] = m->zones[i].bc_type;
+        if (names && name_len > 0) {
+            std::string s = m->zones[i].name;
+            if (s.empty()) s = m->zones[i].kind;
+            s = s.substr(0, (size_t)name_len - 1);
+            std::memcpy(names + i * name_len, s.c_str(), s.size() + 1);
+        }
+    }
+    return FVDG_OK;
+}
+
+int fvdg_mesh_set_zone_boundary_type(fvdg_mesh* m, int32_t zone, uint8_t bc) {
+    if (!m) return fail(FVDG_ERR_ARG, "null mesh");
+    if (bc >= (uint8_t)BoundaryType::COUNT) return fail(FVDG_ERR_ARG, "unknown boundary type");
+    int64_t hit = 0;
+    for (size_t f = 0; f < m->cm.m_faces.size() && f < m->face_zone.size(); ++f)
+        if (m->face_zone[f] == zone && m->cm.m_faces[f].is_boundary()) {
+            m->cm.m_boundaryTypes[f] = (BoundaryType)bc;
+            ++hit;
+        }
+    if (hit == 0) return fail(FVDG_ERR_ARG, "no boundary face in that zone");
+    return FVDG_OK;
+}
+
 int fvdg_mesh_sizes(const fvdg_mesh* m, int64_t* np, int64_t* nc, int64_t* nf) {
     if (!m) return fail(FVDG_ERR_ARG, "null mesh");
     if (np) *np = m->cm.m_points.size();
@@ -478,6 +542,80 @@ int fvdg_project_analytic(const fvdg_mesh* m, int order, int field, const double
                 for (int q = 0; q < Q; ++q) rhs += f[q][v] * phiw[q * N + k];
                 out[5 * k + v] = rhs / (double)diag[k];
             }
+    });
+    return FVDG_OK;
+}
+
+int fvdg_solution_write(const fvdg_mesh* m, int order, const double* U, int field, const double* params,
+                        int n_params, double t, const char* path) {
+    if (!m || !U || !path) return fail(FVDG_ERR_ARG, "null argument");
+    if (order < 0 || order > 5) return fail(FVDG_ERR_ARG, "order must be 0..5");
+    double probe[5];
+    if (field >= 0 && !analytic_state(field, params, n_params, t, {0, 0, 0}, probe))
+        return fail(FVDG_ERR_ARG, "unknown field or too few parameters");
+    std::ofstream fp(path);
+    if (!fp) return fail(FVDG_ERR_ARG, std::string("cannot write ") + path);
+    const SchemeTables T = build_tables(order);
+    const int N = T.N, Q = T.Q;
+    const auto& cm = m->cm;
+    // one line per (cell, volume point): x y z, then (U_h, U_s) per variable, 16 digits
+    // (examples/Vortex/VortexBatch3.cpp:402-415)
+    for (size_t e = 0; e < cm.m_cells.size(); ++e) {
+        const auto& cell = cm.m_cells[e];
+        const double* c = U + e * 5 * N;
+        for (int q = 0; q < Q; ++q) {
+            const vector3f xi = {T.vq_pts[3 * q], T.vq_pts[3 * q + 1], T.vq_pts[3 * q + 2]};
+            const vector3f pos = cell.transform_to_physical(xi);
+            double uh[5] = {0, 0, 0, 0, 0}, us[5] = {0, 0, 0, 0, 0};
+            for (int k = 0; k < N; ++k)
+                for (int v = 0; v < 5; ++v) uh[v] += T.V[q * N + k] * c[5 * k + v];
+            if (field >= 0) analytic_state(field, params, n_params, t, pos, us);
+            fp << std::setprecision(16) << pos[0] << "  " << pos[1] << "  " << pos[2];
+            for (int v = 0; v < 5; ++v) fp << "  " << uh[v] << "  " << us[v];
+            fp << "\n";
+        }
+    }
+    fp.close();
+    if (!fp) return fail(FVDG_ERR_ARG, std::string("write failed: ") + path);
+    return FVDG_OK;
+}
+
+int fvdg_solution_read(const fvdg_mesh* m, int order, const char* path, double* U) {
+    if (!m || !U || !path) return fail(FVDG_ERR_ARG, "null argument");
+    if (order < 0 || order > 5) return fail(FVDG_ERR_ARG, "order must be 0..5");
+    std::ifstream in(path);
+    if (!in) return fail(FVDG_ERR_ARG, std::string("cannot open ") + path);
+    const SchemeTables T = build_tables(order);
+    const int N = T.N, Q = T.Q;
+    const size_t E = m->cm.m_cells.size();
+    // U_h at the volume points (x y z, then U_h U_s pairs), then per mode the diagonal
+    // projection sum_q U_h phi_k w / sum_q phi_k^2 w (examples/Vortex/tools.h:72-129)
+    std::vector<double> uh((size_t)E * Q * 5);
+    std::string line;
+    size_t row = 0;
+    while (row < E * Q && std::getline(in, line)) {
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        std::istringstream iss(line);
+        double x, y, z, a, b;
+        if (!(iss >> x >> y >> z)) return fail(FVDG_ERR_ARG, "malformed solution line " + std::to_string(row + 1));
+        for (int v = 0; v < 5; ++v) {
+            if (!(iss >> a >> b)) return fail(FVDG_ERR_ARG, "malformed solution line " + std::to_string(row + 1));
+            uh[row * 5 + v] = a;
+        }
+        ++row;
+    }
+    if (row != E * Q) return fail(FVDG_ERR_ARG, "solution file has fewer lines than cells x volume points");
+    std::vector<double> den(N, 0.0);
+    for (int k = 0; k < N; ++k)
+        for (int q = 0; q < Q; ++q) den[k] += T.V[q * N + k] * T.V[q * N + k] * T.vq_w[q];
+    parallel_for(64, [&](int64_t ch) {
+        for (size_t e = E * ch / 64; e < E * (ch + 1) / 64; ++e)
+            for (int k = 0; k < N; ++k)
+                for (int v = 0; v < 5; ++v) {
+                    double s = 0.0;
+                    for (int q = 0; q < Q; ++q) s += uh[(e * Q + q) * 5 + v] * T.V[q * N + k] * T.vq_w[q];
+                    U[e * 5 * N + 5 * k + v] = s / den[k];
+                }
     });
     return FVDG_OK;
 }

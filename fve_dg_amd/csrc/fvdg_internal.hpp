@@ -6,10 +6,14 @@
 #include "DG/DG_Basis/DG_Basis.h"
 #include "GaussQuadrature/GaussLegendre.h"
 #include "Mesh/ComputingMesh.h"
+#include "Mesh/FluentMesh.h"
 #include "fvdg.h"
 
 struct fvdg_mesh {
     ComputingMesh cm;
+    // meshes read from a file: per face the file's zone (-1: none) and the zones
+    std::vector<int32_t> face_zone;
+    std::vector<FluentZone> zones;
 };
 
 namespace fvdg {

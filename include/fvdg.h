@@ -91,6 +91,19 @@ int fvdg_mesh_create_box(fvdg_mesh** out, const double lb[3], const double ub[3]
 int fvdg_mesh_create(fvdg_mesh** out, int64_t n_points, const double* points, int64_t n_cells,
                      const int64_t* cell_nodes, int64_t n_faces, const int64_t* face_nodes,
                      const int64_t* face_cells, const int64_t* face_nodes_r);
+/* read a Fluent ASCII mesh file (.msh, like the reference's data/case11.msh): cells other than
+ * tetrahedra are split into tetrahedra around their centroids (quadrilateral faces cut along the
+ * diagonal through their smallest node, so the result is conforming); boundary faces get the
+ * BoundaryType of their zone's Fluent type (inlet -> Inflow, outlet -> Outflow, symmetry ->
+ * Neumann, far field -> Dirichlet, others -> Wall).  No counterpart in the reference, whose
+ * mesh reader is commented out (tests/a_new_test.cpp:16). */
+int fvdg_mesh_read_fluent(fvdg_mesh** out, const char* path);
+/* per face: the file zone of the face (-1: a face created inside a split cell or not from a file) */
+int fvdg_mesh_face_zones(const fvdg_mesh* m, int32_t* zone);
+/* zones of a mesh read from a file: ids, Fluent boundary-condition codes, names (name_len bytes each) */
+int fvdg_mesh_zones(const fvdg_mesh* m, int* n, int32_t* ids, int32_t* bc_types, char* names, int name_len);
+/* set the boundary type of every boundary face of a zone (e.g. FVDG_BC_DIRICHLET, FVDG_BC_PSEUDO3DZ) */
+int fvdg_mesh_set_zone_boundary_type(fvdg_mesh* m, int32_t zone, uint8_t bc);
 int fvdg_mesh_sizes(const fvdg_mesh* m, int64_t* n_points, int64_t* n_cells, int64_t* n_faces);
 /* any output pointer may be NULL */
 int fvdg_mesh_export(const fvdg_mesh* m, double* points, int64_t* cell_nodes, int64_t* face_nodes,
@@ -156,6 +169,15 @@ int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);
  * kernels gives ms[1] = volume, ms[2] = face flux, ms[3] = face test + stage update
  * (per-stage averages). */
 int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms);
+
+/* solution files in the reference's text format (examples/Vortex/VortexBatch3.cpp:402-415): one
+ * line per (cell, volume quadrature point): x y z, then U_h U_s for rho, rho u, rho v, rho w,
+ * rho E, 16 significant digits; U_s is the analytic field (field >= 0) or 0 (field < 0) */
+int fvdg_solution_write(const fvdg_mesh* m, int order, const double* U, int field, const double* params,
+                        int n_params, double t, const char* path);
+/* read such a file back into coefficients (diagonal projection of the U_h point values,
+ * examples/Vortex/tools.h:72-129) */
+int fvdg_solution_read(const fvdg_mesh* m, int order, const char* path, double* U);
 
 /* ---------------- ensembles: many independent problems of uneven size and order ----------------
  * (no counterpart in the reference, whose drivers run one problem at a time.)  Problems of one
