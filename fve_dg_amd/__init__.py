@@ -21,6 +21,7 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 FIELD = {"vortex": 0, "tgv": 1, "const": 2}
 PART_CELLS, PART_INTERNALS, PART_BOUNDARYS, PART_ALL = 1, 2, 4, 7
 BC_DIRICHLET, BC_PSEUDO3DZ = 0, 5
+LIMIT_BOUNDS, LIMIT_POSITIVITY = 1, 2  # PositiveLimiter apply_1, apply_2
 FVDG_BC_DIRICHLET, FVDG_BC_PSEUDO3DZ = BC_DIRICHLET, BC_PSEUDO3DZ
 # status codes of include/fvdg.h
 FVDG_OK, FVDG_ERR_ARG, FVDG_ERR_HIP, FVDG_ERR_MESH = 0, -1, -2, -3
@@ -92,6 +93,10 @@ def lib():
         _sig(L, "fvdg_max_wave_speed", [_vp, _d])
         _sig(L, "fvdg_conservation_sums", [_vp, _d])
         _sig(L, "fvdg_time_stages", [_vp, C.c_double, C.c_int, _d])
+        _sig(L, "fvdg_flow_diagnostics", [_vp, _d])
+        _sig(L, "fvdg_limiter_bounds", [_vp, _d, C.c_int])
+        _sig(L, "fvdg_limiter_apply", [_vp, _d, C.c_int])
+        _sig(L, "fvdg_set_stage_limiter", [_vp, C.c_int, C.c_int])
         _sig(L, "fvdg_ensemble_create", [C.POINTER(_vp), C.c_int, C.c_int, C.POINTER(_vp), C.POINTER(C.c_int),
                                          C.c_int, C.c_double, C.c_int])
         _sig(L, "fvdg_ensemble_destroy", [_vp], None)
@@ -369,6 +374,41 @@ class Solver:
         ms = np.zeros(4)
         _check(lib().fvdg_time_stages(self._h, dt, n_stages, _p(ms)))
         return ms
+
+    DIAGNOSTICS = ("volume", "mass", "kinetic_energy", "enstrophy", "total_energy", "min_rho", "min_p", "max_mach")
+
+    def flow_diagnostics(self):
+        """dict of the device state's integrals (volume, mass, kinetic energy, enstrophy, total
+        energy) and point extremes (min rho, min p, max Mach)"""
+        out = np.zeros(8)
+        _check(lib().fvdg_flow_diagnostics(self._h, _p(out)))
+        return dict(zip(self.DIAGNOSTICS, out.tolist()))
+
+    # ---- limiter (PositiveLimiter.h: constructMinMax, apply / apply_1 / apply_2)
+    def limiter_bounds(self, U_prev=None, neighbour_averages=False):
+        """bounds from U_prev (host array) or, with None, from the device state"""
+        Up = None
+        if U_prev is not None:
+            Up = np.ascontiguousarray(U_prev, dtype=np.float64)
+            if Up.shape != (self.n_cells, self.nv):
+                raise ValueError(f"state shape {Up.shape} != {(self.n_cells, self.nv)}")
+        _check(lib().fvdg_limiter_bounds(self._h, _p(Up), int(bool(neighbour_averages))))
+
+    def limit(self, U=None, parts=LIMIT_BOUNDS | LIMIT_POSITIVITY):
+        """limited copy of U (host array), or, with None, the device state limited in place"""
+        if U is None:
+            _check(lib().fvdg_limiter_apply(self._h, None, parts))
+            return None
+        out = np.array(U, dtype=np.float64, copy=True, order="C")
+        if out.shape != (self.n_cells, self.nv):
+            raise ValueError(f"state shape {out.shape} != {(self.n_cells, self.nv)}")
+        _check(lib().fvdg_limiter_apply(self._h, _p(out), parts))
+        return out
+
+    def set_stage_limiter(self, parts=LIMIT_BOUNDS | LIMIT_POSITIVITY, neighbour_averages=False):
+        """limit every SSP-RK3 stage result (parts 0: off)"""
+        _check(lib().fvdg_set_stage_limiter(self._h, parts, int(bool(neighbour_averages))))
+        return self
 
 
 class Ensemble:

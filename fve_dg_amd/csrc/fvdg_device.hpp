@@ -92,6 +92,13 @@ struct fvdg_ctx {
     DevBuf<double> red;              // reduction partials
     DevBuf<double> edt;              // per-element time step (ensembles); empty: the call's dt
     int64_t n_red = 0;
+    // limiter (PositiveLimiter): per-element extremes of the previous state, staging of
+    // host arrays, and the limiter applied after every SSP-RK3 stage (parts 0: none)
+    DevBuf<double> lim_ext;          // E*10
+    DevBuf<double> lim_buf;          // E*5N, allocated on first host-array call
+    bool lim_bounds = false;         // lim_ext holds bounds of the current mesh/scheme
+    int lim_parts = 0, lim_avg = 0;
+    DevBuf<double> diag;             // flow diagnostics: 8 partials per block
 
     ~fvdg_ctx() {
         if (stream && owns_stream) (void)hipStreamDestroy(stream);

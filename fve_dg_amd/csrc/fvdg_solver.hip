@@ -26,6 +26,8 @@
 #include "fvdg_internal.hpp"
 #include "fvdg_kernels.hpp"
 #include "fvdg_kernels_fast.hpp"
+#include "fvdg_limiter.hpp"
+#include "fvdg_diagnostics.hpp"
 
 using namespace fvdg;
 
@@ -107,6 +109,11 @@ int precision_buffers(fvdg_ctx* c) {
     return FVDG_OK;
 }
 
+// lanes per element of the per-point element kernels (limiter, diagnostics): the power of
+// two >= Q; 256 / lanes elements per block
+inline int limiter_lanes(int Q) { return Q <= 16 ? 16 : Q <= 32 ? 32 : Q <= 64 ? 64 : 128; }
+inline unsigned point_blocks(const fvdg_ctx* c) { return nblocks(c->E, 256 / limiter_lanes(c->tab.Q)); }
+
 int alloc_state(fvdg_ctx* c) {
     if (c->order < 0 || !c->have_mesh) return FVDG_OK;
     const int N = c->tab.N;
@@ -124,6 +131,10 @@ int alloc_state(fvdg_ctx* c) {
     HIP_TRY(hipMemsetAsync(c->UD.p, 0, c->UD.n * sizeof(double), c->stream));
     c->n_red = std::max<int64_t>(nblocks(std::max(c->E, c->F * c->tab.Qf), 256), 1);
     HIP_TRY(c->red.alloc((size_t)c->n_red * 5));
+    HIP_TRY(c->lim_ext.alloc((size_t)c->E * 10));
+    HIP_TRY(c->diag.alloc((size_t)point_blocks(c) * fvdg_diag::kNumDiag));
+    c->lim_buf.release();
+    c->lim_bounds = false;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->have_state = false;
     return FVDG_OK;
@@ -379,6 +390,34 @@ int launch_residual(fvdg_ctx* c, const double* U, int parts, double* ms_vol, dou
     return FVDG_OK;
 }
 
+// ---------------------------------------------------------------- limiter
+
+// lim_ext <- per-element extremes of Uprev (constructMinMax)
+void launch_limit_extrema(fvdg_ctx* c, const double* Uprev, int only_avg) {
+    const int N = c->tab.N, Q = c->tab.Q, L = limiter_lanes(Q), EB = 256 / L;
+    const size_t lds = sizeof(double) * ((size_t)EB * 5 * N + 256);
+    const dim3 grid(nblocks(c->E, EB));
+    switch (L) {
+        case 16: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<16>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
+        case 32: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<32>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
+        case 64: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<64>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
+        default: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<128>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
+    }
+}
+
+// U <- limited U (apply_1 when parts & 1 with the bounds of lim_ext, apply_2 when parts & 2)
+void launch_limit_apply(fvdg_ctx* c, double* U, int parts) {
+    const int N = c->tab.N, Q = c->tab.Q, L = limiter_lanes(Q), EB = 256 / L;
+    const size_t lds = sizeof(double) * ((size_t)EB * 5 * N + (size_t)EB * 10 + 256);
+    const dim3 grid(nblocks(c->E, EB));
+    switch (L) {
+        case 16: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<16>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
+        case 32: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<32>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
+        case 64: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<64>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
+        default: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<128>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
+    }
+}
+
 // One SSP-RK3 step on c->U (Shu & Osher 1988):
 //   U1 = U + dt L(U);  U2 = 3/4 U + 1/4 (U1 + dt L(U1));  U <- 1/3 U + 2/3 (U2 + dt L(U2)),
 // L = -M^{-1} R.  Stage inputs and outputs ping-pong between Ua and Ub so that no
@@ -406,6 +445,11 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
             sv.stop();
             const StageUpdate up{c->U.p, dst[s], A[s], B[s], dt};
             dispatch_faces(c, src[s], FVDG_PART_ALL, true, &up, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
+        }
+        if (c->lim_parts) {  // limit the stage result, bounds from the stage input
+            if (c->lim_parts & FVDG_LIMIT_BOUNDS) launch_limit_extrema(c, src[s], c->lim_avg);
+            launch_limit_apply(c, dst[s], c->lim_parts);
+            c->lim_bounds = (c->lim_parts & FVDG_LIMIT_BOUNDS) != 0;
         }
         HIP_TRY(hipGetLastError());
     }
@@ -739,6 +783,85 @@ int fvdg_conservation_sums(fvdg_ctx* c, double out[5]) {
         for (unsigned b = 0; b < nb; ++b) s += part[b * 5 + v];
         out[v] = s;
     }
+    return FVDG_OK;
+}
+
+int fvdg_limiter_bounds(fvdg_ctx* c, const double* U_prev, int neighbour_averages) {
+    int rc = check_ready(c, U_prev == nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    const double* src = c->U.p;
+    if (U_prev) {
+        const size_t n = (size_t)c->E * 5 * c->tab.N;
+        if (c->lim_buf.n != n) HIP_TRY(c->lim_buf.alloc(n));
+        HIP_TRY(hipMemcpyAsync(c->lim_buf.p, U_prev, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        src = c->lim_buf.p;
+    }
+    launch_limit_extrema(c, src, neighbour_averages != 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->lim_bounds = true;
+    return FVDG_OK;
+}
+
+int fvdg_limiter_apply(fvdg_ctx* c, double* U, int parts) {
+    int rc = check_ready(c, U == nullptr);
+    if (rc) return rc;
+    if (parts < 0 || parts > 3) return fail(FVDG_ERR_ARG, "parts must combine FVDG_LIMIT_BOUNDS and FVDG_LIMIT_POSITIVITY");
+    if ((parts & FVDG_LIMIT_BOUNDS) && !c->lim_bounds)
+        return fail(FVDG_ERR_STATE, "fvdg_limiter_bounds has not been called for this mesh");
+    if (parts == 0) return FVDG_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    double* dst = c->U.p;
+    const size_t n = (size_t)c->E * 5 * c->tab.N;
+    if (U) {
+        if (c->lim_buf.n != n) HIP_TRY(c->lim_buf.alloc(n));
+        HIP_TRY(hipMemcpyAsync(c->lim_buf.p, U, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        dst = c->lim_buf.p;
+    }
+    launch_limit_apply(c, dst, parts);
+    HIP_TRY(hipGetLastError());
+    if (U) HIP_TRY(hipMemcpyAsync(U, dst, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FVDG_OK;
+}
+
+int fvdg_set_stage_limiter(fvdg_ctx* c, int parts, int neighbour_averages) {
+    if (!c) return fail(FVDG_ERR_ARG, "null context");
+    if (parts < 0 || parts > 3) return fail(FVDG_ERR_ARG, "parts must combine FVDG_LIMIT_BOUNDS and FVDG_LIMIT_POSITIVITY");
+    c->lim_parts = parts;
+    c->lim_avg = neighbour_averages != 0;
+    return FVDG_OK;
+}
+
+int fvdg_flow_diagnostics(fvdg_ctx* c, double out[8]) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (!out) return fail(FVDG_ERR_ARG, "null output");
+    HIP_TRY(hipSetDevice(c->device));
+    const int N = c->tab.N, Q = c->tab.Q, L = limiter_lanes(Q), EB = 256 / L;
+    const unsigned nb = point_blocks(c);
+    if ((size_t)nb * fvdg_diag::kNumDiag > c->diag.n) return fail(FVDG_ERR_STATE, "diagnostics buffer too small");
+    const size_t lds = sizeof(double) * ((size_t)EB * 5 * N + (size_t)fvdg_diag::kNumDiag * 256);
+    switch (L) {
+        case 16: hipLaunchKernelGGL(fvdg_diag::k_flow_diagnostics<16>, dim3(nb), dim3(256), lds, c->stream, c->U.p, c->cgeo.p, c->tV.p, c->tG.p, c->tWq.p, c->E, N, Q, c->gamma, c->diag.p); break;
+        case 32: hipLaunchKernelGGL(fvdg_diag::k_flow_diagnostics<32>, dim3(nb), dim3(256), lds, c->stream, c->U.p, c->cgeo.p, c->tV.p, c->tG.p, c->tWq.p, c->E, N, Q, c->gamma, c->diag.p); break;
+        case 64: hipLaunchKernelGGL(fvdg_diag::k_flow_diagnostics<64>, dim3(nb), dim3(256), lds, c->stream, c->U.p, c->cgeo.p, c->tV.p, c->tG.p, c->tWq.p, c->E, N, Q, c->gamma, c->diag.p); break;
+        default: hipLaunchKernelGGL(fvdg_diag::k_flow_diagnostics<128>, dim3(nb), dim3(256), lds, c->stream, c->U.p, c->cgeo.p, c->tV.p, c->tG.p, c->tWq.p, c->E, N, Q, c->gamma, c->diag.p); break;
+    }
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part((size_t)nb * fvdg_diag::kNumDiag);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->diag.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double r[8] = {0, 0, 0, 0, 0, part[5], part[6], part[7]};
+    for (unsigned b = 0; b < nb; ++b) {
+        const double* p = &part[(size_t)b * fvdg_diag::kNumDiag];
+        for (int i = 0; i < 5; ++i) r[i] += p[i];
+        r[5] = std::min(r[5], p[5]);
+        r[6] = std::min(r[6], p[6]);
+        r[7] = std::max(r[7], p[7]);
+    }
+    for (int i = 0; i < 8; ++i) out[i] = r[i];
     return FVDG_OK;
 }
 

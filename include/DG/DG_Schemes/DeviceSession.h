@@ -115,6 +115,13 @@ public:
     }
     void mass(double* M) { check(fvdg_mass_diagonal(ctx(), M), "fvdg_mass_diagonal"); }
     void conservation_sums(double out[5]) { check(fvdg_conservation_sums(ctx(), out), "fvdg_conservation_sums"); }
+    void limiter_bounds(const double* U_prev, bool neighbour_averages) {
+        check(fvdg_limiter_bounds(ctx(), U_prev, neighbour_averages ? 1 : 0), "fvdg_limiter_bounds");
+    }
+    void limiter_apply(double* U, int parts) { check(fvdg_limiter_apply(ctx(), U, parts), "fvdg_limiter_apply"); }
+    void set_stage_limiter(int parts, bool neighbour_averages) {
+        check(fvdg_set_stage_limiter(ctx(), parts, neighbour_averages ? 1 : 0), "fvdg_set_stage_limiter");
+    }
 
 private:
     bool same_boundary_types(const ComputingMesh& cm) const { return cm.m_boundaryTypes == bc_; }

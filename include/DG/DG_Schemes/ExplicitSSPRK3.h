@@ -45,6 +45,9 @@ public:
         session().mass(M.data());
         return M;
     }
+    // limit every stage result with PositiveLimiter's steps (FVDG_LIMIT_BOUNDS: apply_1,
+    // FVDG_LIMIT_POSITIVITY: apply_2; 0: off), bounds from the stage's input state
+    void set_limiter(int parts, bool neighbour_averages = false) { session().set_stage_limiter(parts, neighbour_averages); }
     // cfl * h_min / ((2p+1) lambda_max) for the current device state
     Scalar stable_dt(Scalar cfl = 0.3) { return session().stable_dt(cfl); }
 

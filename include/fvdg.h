@@ -170,6 +170,32 @@ int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);
  * (per-stage averages). */
 int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms);
 
+/* flow diagnostics of the device state, integrals at the volume quadrature points with the
+ * weights w_q |det A| of the reference's err_integral (examples/Vortex/VortexBatch3.cpp:349-366):
+ * out[0] volume, [1] mass (rho), [2] kinetic energy (rho |u|^2 / 2), [3] enstrophy
+ * (rho |curl u|^2 / 2), [4] total energy (rho E); [5] min rho, [6] min p, [7] max Mach over the
+ * points.  [2] and [3] divided by rho_0 * volume are the Taylor-Green vortex's E_k and epsilon
+ * series (no counterpart in the reference). */
+int fvdg_flow_diagnostics(fvdg_ctx* c, double out[8]);
+
+/* ---------------- limiter: PositiveLimiter<Order, QuadC, OnlyNeigbAvg> of
+ * include/DG/DG_Schemes/PositiveLimiter.h:12-261, sampling each element at the points of
+ * its order's tetrahedron rule.  Cell averages (mode 0) are never changed. */
+#define FVDG_LIMIT_BOUNDS 1     /* apply_1 (:135-170): point values into the neighbourhood bounds */
+#define FVDG_LIMIT_POSITIVITY 2 /* apply_2 (:171-259): density, then pressure, kept positive */
+/* constructMinMax (:62-119): bounds from U_prev (host, n_cells x 5N; NULL = the device
+ * state), from the point values of each cell and its face neighbours, or from their
+ * averages when neighbour_averages != 0 (OnlyNeigbAvg = true) */
+int fvdg_limiter_bounds(fvdg_ctx* c, const double* U_prev, int neighbour_averages);
+/* apply (parts = 3), apply_1 (1) or apply_2 (2) on U (host, n_cells x 5N, limited in
+ * place; NULL = the device state); FVDG_LIMIT_BOUNDS needs fvdg_limiter_bounds first */
+int fvdg_limiter_apply(fvdg_ctx* c, double* U, int parts);
+/* limit every SSP-RK3 stage result (fvdg_ssprk3_step / _stage) with bounds from that
+ * stage's input state; parts 0 switches it off (the default).  No counterpart in the
+ * reference's explicit drivers, which call the limiter by hand after implicit solves
+ * (examples/Eular2D/Sedov/Sedov.cpp:188-189). */
+int fvdg_set_stage_limiter(fvdg_ctx* c, int parts, int neighbour_averages);
+
 /* solution files in the reference's text format (examples/Vortex/VortexBatch3.cpp:402-415): one
  * line per (cell, volume quadrature point): x y z, then U_h U_s for rho, rho u, rho v, rho w,
  * rho E, 16 significant digits; U_s is the analytic field (field >= 0) or 0 (field < 0) */

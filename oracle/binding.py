@@ -55,6 +55,8 @@ class Oracle:
         _sig(L, "or_vortex_state", [_d, C.c_double, _d, _d], None)
         _sig(L, "or_project_vortex", [C.c_int, _d, C.c_double, C.c_int64, _d, C.c_int64, _i64, _d])
         _sig(L, "or_max_threads", [])
+        _sig(L, "or_limiter_bounds", [C.c_int, C.c_int64, C.c_int64, _i64, _d, C.c_int, _d, _d])
+        _sig(L, "or_limiter_apply", [C.c_int, C.c_int64, _d, _d, C.c_double, C.c_int, _d])
         self.L = L
 
     # -------------------------------------------------- pieces
@@ -153,6 +155,27 @@ class Oracle:
 
     def max_threads(self):
         return self.L.or_max_threads()
+
+    # -------------------------------------------------- limiter (PositiveLimiter.h)
+    def limiter_bounds(self, m, order, U_prev, only_avg=False):
+        """per-cell (max, min) bounds of constructMinMax, each (E, 5)"""
+        fc = np.ascontiguousarray(m["face_cells"], dtype=np.int64)
+        U = np.ascontiguousarray(U_prev, dtype=np.float64)
+        E = U.shape[0]
+        mx, mn = np.zeros((E, 5)), np.zeros((E, 5))
+        rc = self.L.or_limiter_bounds(order, E, len(fc), _p(fc, C.c_int64), _p(U), int(only_avg), _p(mx), _p(mn))
+        if rc:
+            raise RuntimeError(f"or_limiter_bounds failed ({rc})")
+        return mx, mn
+
+    def limiter_apply(self, order, U, bounds, gamma=1.4, parts=3):
+        """apply_1 (parts & 1) then apply_2 (parts & 2) on a copy of U"""
+        out = np.array(U, dtype=np.float64, copy=True, order="C")
+        mx, mn = (np.ascontiguousarray(b, dtype=np.float64) for b in bounds)
+        rc = self.L.or_limiter_apply(order, out.shape[0], _p(mx), _p(mn), gamma, parts, _p(out))
+        if rc:
+            raise RuntimeError(f"or_limiter_apply failed ({rc})")
+        return out
 
 
 def _args(m):

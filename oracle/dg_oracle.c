@@ -25,10 +25,16 @@
  *                      examples/Vortex/123/VortexBatch.cpp:189-197 generalised to 3 stages)
  *   or_vortex_state, or_project
  *                      examples/Vortex/problem.h:17-110, include/DG/DG_Basis/DG_Basis_Evaluator.h:33-59
+ *   or_limiter_bounds, or_limiter_apply
+ *                      include/DG/DG_Schemes/PositiveLimiter.h:62-119 (constructMinMax), :135-170
+ *                      (apply_1), :171-198 (apply_21), :199-259 (apply_22); that header cannot be
+ *                      compiled here (Mesh/Mesh.h reaches CGAL and Boost), so these are pinned by
+ *                      the limiter's invariants in tests/test_limiter.py
  *
  * Layouts: state U[e*5N + 5k + v] (reference LongVector<5N>); cells[4e+a]; faces[3f+k];
  * face_cells[2f+s] with -1 for a boundary side; face_nodes_r[3f+k] = nodes seen from side 1.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -573,6 +579,146 @@ int or_project_vortex(int order, const double* P, double t, int64_t np, const do
     }
     free(phi);
     free(diag);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ limiter
+ * PositiveLimiter<Order, QuadC, OnlyNeigbAvg> of include/DG/DG_Schemes/PositiveLimiter.h,
+ * with QuadC the order's automatic tetrahedron rule (the rule the reference's drivers pass,
+ * examples/Eular2D/Sedov/Sedov.cpp:59,92).  Point values use the tetrahedron rule's points. */
+
+/* values of the 5 variables of one element at the rule's points: out[q*5 + k] */
+static void lim_point_values(int order, const double* c, double* out) {
+    const double* pts;
+    const double* w;
+    const int Q = or_tet_rule(order, &pts, &w), N = or_num_basis(order);
+    double phi[64];
+    for (int q = 0; q < Q; ++q) {
+        or_basis(order, pts[3 * q], pts[3 * q + 1], pts[3 * q + 2], phi, NULL);
+        for (int k = 0; k < 5; ++k) {
+            double v = 0.0;
+            for (int l = 0; l < N; ++l) v += phi[l] * c[k + 5 * l];
+            out[q * 5 + k] = v;
+        }
+    }
+}
+
+/* constructMinMax (PositiveLimiter.h:62-119): per face, the extremes of both neighbours'
+ * (one neighbour's for a boundary face) values of Uprev at the rule's points, or of their
+ * averages (only_avg), folded into per-cell bounds.  The face's lower bound follows the
+ * statement min_U[k] = min(max_U[k], min(val1, val2)) of :98, which leaves the minimum over
+ * the two cells at the LAST point of the rule. */
+int or_limiter_bounds(int order, int64_t nc, int64_t nf, const int64_t* face_cells, const double* Uprev, int only_avg,
+                      double* cmax, double* cmin) {
+    const double* pts;
+    const double* w;
+    const int Q = or_tet_rule(order, &pts, &w), NV = 5 * or_num_basis(order);
+    if (Q == 0) return -1;
+    double* vals = (double*)malloc(sizeof(double) * (size_t)nc * Q * 5);
+    if (!vals) return -2;
+    for (int64_t e = 0; e < nc; ++e) lim_point_values(order, Uprev + e * NV, vals + e * Q * 5);
+    for (int64_t i = 0; i < nc * 5; ++i) cmax[i] = -DBL_MAX, cmin[i] = DBL_MAX;
+    for (int64_t f = 0; f < nf; ++f) {
+        const int64_t c1 = face_cells[2 * f], c2 = face_cells[2 * f + 1] < 0 ? c1 : face_cells[2 * f + 1];
+        double mx[5], mn[5];
+        for (int k = 0; k < 5; ++k) mx[k] = -DBL_MAX, mn[k] = DBL_MAX;
+        if (only_avg) {
+            for (int k = 0; k < 5; ++k) {
+                const double a = Uprev[c1 * NV + k], b = Uprev[c2 * NV + k];
+                mx[k] = a > b ? a : b;
+                mn[k] = a < b ? a : b;
+            }
+        } else {
+            for (int q = 0; q < Q; ++q)
+                for (int k = 0; k < 5; ++k) {
+                    const double a = vals[(c1 * Q + q) * 5 + k], b = vals[(c2 * Q + q) * 5 + k];
+                    const double hi = a > b ? a : b, lo = a < b ? a : b;
+                    mx[k] = mx[k] > hi ? mx[k] : hi;
+                    mn[k] = mx[k] < lo ? mx[k] : lo;
+                }
+        }
+        for (int k = 0; k < 5; ++k) {
+            if (mx[k] > cmax[c1 * 5 + k]) cmax[c1 * 5 + k] = mx[k];
+            if (mn[k] < cmin[c1 * 5 + k]) cmin[c1 * 5 + k] = mn[k];
+            if (face_cells[2 * f + 1] >= 0) {
+                if (mx[k] > cmax[c2 * 5 + k]) cmax[c2 * 5 + k] = mx[k];
+                if (mn[k] < cmin[c2 * 5 + k]) cmin[c2 * 5 + k] = mn[k];
+            }
+        }
+    }
+    free(vals);
+    return 0;
+}
+
+static double lim_pressure(const double U[5], double gamma) { /* PositiveLimiter.h:121-132 */
+    if (U[0] <= 1e-16) return -1.0;
+    const double ke = (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / (2.0 * U[0]);
+    return (gamma - 1.0) * (U[4] - ke);
+}
+
+/* apply_1 (bounds, :135-170) when parts & 1, then apply_21 (density, :171-198) and apply_22
+ * (pressure, :199-259) when parts & 2, cell by cell on U in place.  Cell averages are the
+ * mode-0 coefficients (phi_0 = 1, the other modes have zero mean). */
+int or_limiter_apply(int order, int64_t nc, const double* cmax, const double* cmin, double gamma, int parts,
+                     double* U) {
+    const double* pts;
+    const double* w;
+    const int Q = or_tet_rule(order, &pts, &w), N = or_num_basis(order), NV = 5 * N;
+    if (Q == 0) return -1;
+    double vals[128 * 5];
+    for (int64_t e = 0; e < nc; ++e) {
+        double* c = U + e * NV;
+        if (parts & 1) {
+            lim_point_values(order, c, vals);
+            for (int k = 0; k < 5; ++k) {
+                const double avg = c[k], gmin = cmin[e * 5 + k], gmax = cmax[e * 5 + k];
+                double vmin = vals[k], vmax = vals[k];
+                for (int q = 1; q < Q; ++q) {
+                    const double v = vals[q * 5 + k];
+                    if (v < vmin) vmin = v;
+                    if (v > vmax) vmax = v;
+                }
+                if (vmin < gmin || vmax > gmax) {
+                    const double tmin = (avg != vmin) ? (avg - gmin) / (avg - vmin + 1e-32) : 0.0;
+                    const double tmax = (avg != vmax) ? (gmax - avg) / (vmax - avg + 1e-32) : 0.0;
+                    double th = tmin < tmax ? tmin : tmax;
+                    if (1.0 < th) th = 1.0;
+                    for (int l = 1; l < N; ++l) c[k + 5 * l] *= th;
+                }
+            }
+        }
+        if (parts & 2) {
+            const double eps = 1e-16;
+            lim_point_values(order, c, vals);
+            double rmin = vals[0];
+            for (int q = 1; q < Q; ++q)
+                if (vals[q * 5] < rmin) rmin = vals[q * 5];
+            if (rmin < eps) {
+                const double ravg = c[0] > eps ? c[0] : eps;
+                const double num = ravg - eps, den = ravg - rmin;
+                double th = (den < 1e-32) ? 0.0 : (num / den < 1.0 ? num / den : 1.0);
+                for (int l = 1; l < N; ++l) c[5 * l] *= th;
+            }
+            lim_point_values(order, c, vals);
+            double avg[5], thp = 1.0;
+            for (int k = 0; k < 5; ++k) avg[k] = c[k];
+            for (int q = 0; q < Q; ++q) {
+                const double* P = vals + q * 5;
+                if (lim_pressure(P, gamma) >= 0) continue;
+                double lo = 0.0, hi = 1.0, mid = 0.5;
+                for (int it = 0; it < 100; ++it) {
+                    mid = 0.5 * (lo + hi);
+                    double Um[5];
+                    for (int k = 0; k < 5; ++k) Um[k] = (1.0 - mid) * avg[k] + mid * P[k];
+                    if (lim_pressure(Um, gamma) < 0) hi = mid;
+                    else lo = mid;
+                }
+                if (mid < thp) thp = mid;
+            }
+            for (int k = 0; k < 5; ++k)
+                for (int l = 1; l < N; ++l) c[k + 5 * l] *= thp;
+        }
+    }
     return 0;
 }
 

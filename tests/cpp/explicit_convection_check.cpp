@@ -2,8 +2,8 @@
 // OrthHexMesh + split_hex6_scan + ComputingMesh, tags boundaries like the vortex
 // example (examples/Vortex/tools.h:22-31), defines the exact-solution hooks of
 // base/exact.h, projects the vortex with DGBasisEvaluator::func2coef, and calls
-// ExplicitConvection<P>::eval / eval_cells / eval_internals / eval_boundarys and the
-// ExplicitSSPRK3 driver.  Everything it computes is written to a binary file that
+// ExplicitConvection<P>::eval / eval_cells / eval_internals / eval_boundarys, the
+// ExplicitSSPRK3 driver and PositiveLimiter (constructMinMax, apply, apply_1, apply_2).  Everything it computes is written to a binary file that
 // tests/test_cpp_dropin.py checks against the CPU oracle.
 //
 //   explicit_convection_check OUT.bin
@@ -14,6 +14,7 @@
 
 #include "DG/DG_Schemes/ExplicitConvection.h"
 #include "DG/DG_Schemes/ExplicitSSPRK3.h"
+#include "DG/DG_Schemes/PositiveLimiter.h"
 
 // ---- the vortex of examples/Vortex/problem.h:7-110 (centre (4, 4.5), free stream (1, 0.5, 0), strength 5)
 namespace {
@@ -157,6 +158,38 @@ void periodic_case(Writer& w, int flux) {
     w.vec(U1);
 }
 
+// limiter on a periodic jittered cube, on a sharpened vortex: point bounds with apply, and
+// neighbour-average bounds with apply_1 then apply_2 (as examples/Eular2D/Sedov/Sedov.cpp:92,188-189)
+template <uInt P>
+void limiter_case(Writer& w) {
+    constexpr uInt N = DGBasisEvaluator<P>::NumBasis;
+    OrthHexMesh g({0, 0, 0}, {10, 10, 10}, {3, 3, 3});
+    g.split_hex6_scan();
+    g.make_periodic({0, 0, 0}, {10, 10, 10}, {true, true, true});
+    g.jitter_points(0.1, 9, {0, 0, 0}, {10, 10, 10}, {10.0 / 3, 10.0 / 3, 10.0 / 3}, {true, true, true});
+    ComputingMesh cm(g);
+    const auto Uprev = project<P>(cm, 0.0);
+    auto U = Uprev;
+    for (size_t e = 0; e < U.size(); ++e)
+        for (uInt k = 5; k < 5 * N; ++k) U[e][k] *= 6.0 * (1.0 + 0.5 * std::sin(0.7 * e + k));
+    auto A = U;
+    PositiveLimiter<P> lim(cm, 1.4);
+    lim.constructMinMax(A, Uprev);
+    lim.apply(A, Uprev);
+    auto B = U;
+    using QuadC = typename AutoQuadSelector<P, GaussLegendreTet::Auto>::type;
+    PositiveLimiter<P, QuadC, true> lim2(cm, 1.4);
+    lim2.constructMinMax(B, Uprev);
+    lim2.apply_1(B, Uprev);
+    lim2.apply_2(B, Uprev);
+    w.i64(P);
+    w.mesh(cm);
+    w.vec(Uprev);
+    w.vec(U);
+    w.vec(A);
+    w.vec(B);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -166,10 +199,11 @@ int main(int argc, char** argv) {
     }
     try {
         Writer w(argv[1]);
-        w.i64(3);  // number of cases
+        w.i64(4);  // number of cases
         boundary_case<1>(w, FVDG_FLUX_LF);
         boundary_case<3>(w, FVDG_FLUX_ROE);
         periodic_case<2>(w, FVDG_FLUX_ROE);
+        limiter_case<2>(w);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "explicit_convection_check: %s\n", e.what());
         return 2;

@@ -8,7 +8,9 @@ runs on the GPU and writes its inputs and results; they are checked here against
 * stable_dt = 0.3 h_min / ((2p+1) lambda_max) recomputed from the traces;
 * two driver steps with Dirichlet data refreshed at the stage times t, t+dt, t+dt/2 equal the
   Shu-Osher composition of oracle residuals at those times;
-* a periodic jittered mesh with gamma = 5/3 (MonatomicFluxC) and the Roe flux.
+* a periodic jittered mesh with gamma = 5/3 (MonatomicFluxC) and the Roe flux;
+* PositiveLimiter<2> (constructMinMax + apply) and PositiveLimiter<2, QuadC, true>
+  (constructMinMax + apply_1 + apply_2) on a sharpened vortex equal the oracle's limiter.
 """
 import subprocess
 from pathlib import Path
@@ -103,7 +105,7 @@ def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     r = subprocess.run([str(EXE), str(out)], capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stderr
     rd = Reader(out)
-    assert rd.i64() == 3
+    assert rd.i64() == 4
     # ---- Dirichlet / Pseudo3DZ boxes, P1 LF and P3 Roe
     for _ in range(2):
         order, flux_id = rd.i64(), rd.i64()
@@ -154,4 +156,14 @@ def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     assert (m["face_cells"][:, 1] >= 0).all() and (m["face_nodes_r"] != m["face_nodes"]).any()
     assert rel_err(R, oracle.residual(m, order, U, flux="roe", gamma=5 / 3)) < 1e-12
     assert rel_err(U1, oracle.ssprk3(m, order, U, dt, 1, flux="roe", gamma=5 / 3)) < 1e-11
+    # ---- limiter, P2
+    order = rd.i64()
+    m, _ = rd.mesh()
+    E, N = len(m["cells"]), F.num_basis(order)
+    Uprev, U, A, B = (rd.f64(E * 5 * N).reshape(E, -1) for _ in range(4))
+    expA = oracle.limiter_apply(order, U, oracle.limiter_bounds(m, order, Uprev), parts=3)
+    bB = oracle.limiter_bounds(m, order, Uprev, only_avg=True)
+    expB = oracle.limiter_apply(order, oracle.limiter_apply(order, U, bB, parts=1), bB, parts=2)
+    assert rel_err(A, U) > 1e-3 and rel_err(B, U) > 1e-3  # the limiter acted
+    assert rel_err(A, expA) < 1e-12 and rel_err(B, expB) < 1e-12
     assert rd.o == len(rd.b)
