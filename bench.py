@@ -171,6 +171,12 @@ def run_single(args, w):
         res[prec] = dict(ms=ms, wall=wall, U=U, value=dof / (ms[0] * 1e-3))
         print(f"[bench] {prec}: {ms[0]:.4f} ms/stage, {res[prec]['value']:.4g} DOF-updates/s", file=sys.stderr,
               flush=True)
+    # the same FP64 stages with the limiter (apply_1 + apply_2, point bounds) after each stage
+    s.set_scheme(order, flux=flux, precision="fp64")
+    s.upload(U0)
+    s.set_stage_limiter(F.LIMIT_BOUNDS | F.LIMIT_POSITIVITY)
+    ms_lim = s.time_stages(dt, 3 * args.steps)
+    s.set_stage_limiter(0)
     s.close()
     f64, f32 = res["fp64"], res["fp32"]
     dev = float(np.sqrt(((f32["U"] - f64["U"]) ** 2).sum() / (f64["U"] ** 2).sum()))
@@ -196,6 +202,9 @@ def run_single(args, w):
                   "speedup_vs_fp64": round(f32["value"] / f64["value"], 4),
                   "l2_rel_deviation_vs_fp64": dev, "after_steps": args.warmup + args.steps,
                   "roofline": kernel_roofline(args.workload, f32["ms"], E, Q, Qf, N, "fp32")},
+        "limited": {"limiter": "PositiveLimiter apply_1 + apply_2 after every stage (fp64)",
+                    "ms_per_step": round(3 * ms_lim[0], 4),
+                    "overhead_vs_fp64": round(ms_lim[0] / f64["ms"][0] - 1, 4)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, order, flux, w["field"], L)

@@ -1,0 +1,109 @@
+"""Long-run validation of the device solver: FP64 against the mixed-precision path, and against
+the analytic solution where one exists.
+
+    python -m fve_dg_amd.validate tgv     [--n 40 --order 4 --steps 1000 --every 50]
+    python -m fve_dg_amd.validate vortex  [--n 60 --order 3 --steps 500 --every 100 --jitter 0.1]
+
+tgv:    compressible Taylor-Green vortex on the periodic cube [0, 2pi]^3 (gamma 1.4, rho0 = V0 = 1,
+        Mach 0.1), Roe flux; every `every` steps the kinetic energy E_k and enstrophy epsilon
+        (both per rho0 |Omega|, from fvdg_flow_diagnostics) of each precision; reported: both
+        series and their largest relative deviation, and the final states' relative L2 deviation.
+vortex: isentropic vortex (strength 5, centre (5, 5), free stream (1, 1, 0)) on the periodic
+        cube [0, 10]^3 with jittered vertices, Roe flux; reported: per-variable relative L2 error
+        against the translated exact vortex (err_integral of examples/Vortex/VortexBatch3.cpp:
+        349-366) for each precision, and the FP32-vs-FP64 relative L2 deviation.
+The time step is fixed from the initial state (CFL 0.3).  Prints one JSON object on stdout and
+a progress line per chunk on stderr.
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import Mesh, Solver, num_basis
+
+VORTEX = [1.4, 5.0, 5.0, 5.0, 1.0, 1.0, 10.0, 10.0]
+TGV = [1.4, 1.0, 1.0, 0.1]  # gamma, rho0, V0, Mach
+DEFAULTS = {
+    "tgv": dict(n=40, order=4, steps=1000, every=50, jitter=0.0),
+    "vortex": dict(n=60, order=3, steps=500, every=100, jitter=0.1),
+}
+
+
+def run(kind, n, order, steps, every, jitter, flux="roe", cfl=0.3, precisions=("fp64", "fp32"), device=0):
+    L = 2 * math.pi if kind == "tgv" else 10.0
+    field, params = ("tgv", TGV) if kind == "tgv" else ("vortex", VORTEX)
+    mesh = Mesh.box(n, ub=(L, L, L), periodic=7, jitter=jitter, seed=60)
+    U0 = mesh.project(order, field, params)
+    s = Solver(device)
+    s.set_scheme(order, flux=flux, precision="fp64").set_mesh(mesh)
+    s.upload(U0)
+    dt = s.stable_dt(cfl)
+    vol = L ** 3
+    out = dict(kind=kind, tets=mesh.n_cells, order=order, dofs=5 * num_basis(order) * mesh.n_cells, flux=flux,
+               steps=steps, dt=dt, t_end=steps * dt, jitter=jitter)
+    states = {}
+    for prec in precisions:
+        s.set_scheme(order, flux=flux, precision=prec)
+        s.upload(U0)
+        series = []
+
+        def record(t):
+            d = s.flow_diagnostics()
+            series.append([t, d["kinetic_energy"] / (params[1] * vol) if kind == "tgv" else d["kinetic_energy"],
+                           d["enstrophy"] / (params[1] * vol) if kind == "tgv" else d["enstrophy"],
+                           d["mass"], d["total_energy"], d["min_rho"], d["min_p"]])
+
+        record(0.0)
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            s.ssprk3(dt, k)
+            done += k
+            record(done * dt)
+            print(f"[validate] {kind} {prec}: step {done}/{steps}, E_k {series[-1][1]:.10g}, "
+                  f"enstrophy {series[-1][2]:.10g}", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        U = s.download()
+        if not np.isfinite(U).all():
+            raise RuntimeError(f"non-finite state ({prec})")
+        states[prec] = U
+        rec = dict(series_columns=["t", "kinetic_energy", "enstrophy", "mass", "total_energy", "min_rho", "min_p"],
+                   series=series, wall_s=round(wall, 3))
+        if kind == "vortex":
+            rec["analytic_l2_rel"] = mesh.analytic_error(order, field, params, steps * dt, U).tolist()
+        out[prec] = rec
+    s.close()
+    if "fp64" in states and "fp32" in states:
+        a, b = states["fp32"], states["fp64"]
+        out["l2_rel_deviation_fp32_vs_fp64"] = float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+        s64, s32 = np.array(out["fp64"]["series"]), np.array(out["fp32"]["series"])
+        out["series_max_rel_deviation"] = {
+            name: float(np.max(np.abs(s32[:, i] - s64[:, i]) / np.maximum(np.abs(s64[:, i]), 1e-300)))
+            for i, name in ((1, "kinetic_energy"), (2, "enstrophy"), (3, "mass"), (4, "total_energy"))}
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("kind", choices=sorted(DEFAULTS))
+    ap.add_argument("--n", type=int)
+    ap.add_argument("--order", type=int)
+    ap.add_argument("--steps", type=int)
+    ap.add_argument("--every", type=int)
+    ap.add_argument("--jitter", type=float)
+    ap.add_argument("--flux", default="roe", choices=["lf", "roe"])
+    args = ap.parse_args(argv)
+    cfg = dict(DEFAULTS[args.kind])
+    for k in cfg:
+        if getattr(args, k) is not None:
+            cfg[k] = getattr(args, k)
+    print(json.dumps(run(args.kind, flux=args.flux, **cfg)))
+
+
+if __name__ == "__main__":
+    main()
