@@ -18,7 +18,9 @@
 //   lane group l/16 feeds exactly those points: no data movement between the two
 //   products.  The 16-element coefficient tile (contiguous in the element-major
 //   layout) is staged once through LDS with full-width coalesced loads, and the
-//   result goes back the same way.  Accumulators live in AGPRs.
+//   result goes back the same way.  Accumulators live in AGPRs.  The EXTREMA form
+//   also writes the limiter's per-element extremes of U (max over the points and the
+//   last point's values), read off the interpolated point values it already holds.
 //
 // Faces (no atomics anywhere; every output has one fixed summation order):
 //   k_face_flux_lds + k_face_test_lds  two passes (default): each face's traces and
@@ -30,6 +32,7 @@
 //   combination) when used inside the time stepper, so the total residual never goes
 //   to memory.
 #pragma once
+#include <float.h>
 #include <type_traits>
 
 #include "fvdg_kernels.hpp"
@@ -104,6 +107,7 @@ struct VolArgs {
     const T* GA;         // QPAD*3*JPAD G[q][d][j], zero padded
     const double* wq;    // QPAD, zero padded
     const double* Gs;    // N*3: sum_q w_q d phi_j / d xi_d (FP32 path only)
+    double* ext = nullptr;  // EXTREMA: E*10 limiter extremes of U (max over points, last point)
 };
 
 // k_vol_mfma<ORDER,T>: T = double is the FP64 path.  T = float splits off the parts
@@ -113,12 +117,13 @@ struct VolArgs {
 // flux's exact contribution -|J| (F(c_0) A^{-1}) . sum_q w_q grad phi_j is added in
 // FP64.  FP32 rounding then scales with the variation of the state inside the cell,
 // and a uniform state has an FP64-exact residual.
-template <int ORDER, typename T>
+template <int ORDER, typename T, bool EXTREMA = false>
 __global__ __launch_bounds__(64) void k_vol_mfma(VolArgs<T> a) {
     using S = VolShape<ORDER>;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     constexpr bool SPLIT = !std::is_same<T, double>::value;
+    static_assert(!(SPLIT && EXTREMA), "limiter extremes need the FP64 point values");
     constexpr int N = S::N, Q = S::Q, NV = S::NV, QT = S::QT, KS = S::KS, JT = S::JT;
     constexpr int KPAD = S::KPAD, JPAD = S::JPAD, STR = S::STRIDE;
     __shared__ T sC[16 * STR];
@@ -163,6 +168,9 @@ __global__ __launch_bounds__(64) void k_vol_mfma(VolArgs<T> a) {
     for (int v = 0; v < 5; ++v)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) racc[v][jt] = (acc_t){0, 0, 0, 0};
+    // EXTREMA: this lane's running maximum over its points (the last point's values are
+    // stored as soon as they are known)
+    double emax[5] = {-DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
 
 #pragma unroll 1
     for (int t = 0; t < QT; ++t) {
@@ -189,6 +197,13 @@ __global__ __launch_bounds__(64) void k_vol_mfma(VolArgs<T> a) {
                 double Uq[5];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) Uq[v] = c0[v] + (double)uacc[v][r];
+                if (EXTREMA) {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) emax[v] = Uq[v] > emax[v] ? Uq[v] : emax[v];
+                    if (q == Q - 1 && e < a.E)
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) a.ext[e * 10 + 5 + v] = Uq[v];
+                }
                 double F[5][3];
                 euler_flux(Uq, a.gamma, F);
                 const double jw = -a.wq[q] * det;
@@ -216,6 +231,19 @@ __global__ __launch_bounds__(64) void k_vol_mfma(VolArgs<T> a) {
                 }
             }
         }
+    }
+
+    if (EXTREMA) {  // the 4 lanes of an element (grp 0..3) hold its 16-point tiles' rows
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            double x = __shfl_xor(emax[v], 16, 64);
+            emax[v] = x > emax[v] ? x : emax[v];
+            x = __shfl_xor(emax[v], 32, 64);
+            emax[v] = x > emax[v] ? x : emax[v];
+        }
+        if (e < a.E && grp == 0)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) a.ext[e * 10 + v] = emax[v];
     }
 
     // 5. result R_v[j = 16 jt + row(r, grp)][element col] -> LDS -> coalesced store

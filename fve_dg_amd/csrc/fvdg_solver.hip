@@ -236,26 +236,34 @@ struct StageUpdate {
 inline bool mixed(const fvdg_ctx* c) { return c->precision != FVDG_PREC_FP64 && !c->simple; }
 
 template <int P, typename T>
-void launch_vol_t(fvdg_ctx* c, const double* U, double* R, const T* VA, const T* GA) {
+void launch_vol_t(fvdg_ctx* c, const double* U, double* R, const T* VA, const T* GA, double* ext) {
     VolArgs<T> v{c->E, c->gamma, U, R, c->cgeo.p, VA, GA, c->tWqa.p, c->tGs.p};
+    if constexpr (std::is_same<T, double>::value) {
+        if (ext) {
+            v.ext = ext;
+            hipLaunchKernelGGL((k_vol_mfma<P, T, true>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, v);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_vol_mfma<P, T>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, v);
 }
 template <int P>
-void launch_vol_fast(fvdg_ctx* c, const double* U) {
+void launch_vol_fast(fvdg_ctx* c, const double* U, double* ext) {
     if (mixed(c))
-        launch_vol_t<P, float>(c, U, c->R.p, c->tVA32.p, c->tGA32.p);
+        launch_vol_t<P, float>(c, U, c->R.p, c->tVA32.p, c->tGA32.p, nullptr);
     else
-        launch_vol_t<P, double>(c, U, c->R.p, c->tVA.p, c->tGA.p);
+        launch_vol_t<P, double>(c, U, c->R.p, c->tVA.p, c->tGA.p, ext);
 }
-// volume part of the residual of U into c->R
-void dispatch_vol(fvdg_ctx* c, const double* U) {
+// volume part of the residual of U into c->R; ext (FP64 path only, else ignored): also the
+// limiter's per-element extremes of U
+void dispatch_vol(fvdg_ctx* c, const double* U, double* ext = nullptr) {
     switch (c->order) {
-        case 0: launch_vol_fast<0>(c, U); break;
-        case 1: launch_vol_fast<1>(c, U); break;
-        case 2: launch_vol_fast<2>(c, U); break;
-        case 3: launch_vol_fast<3>(c, U); break;
-        case 4: launch_vol_fast<4>(c, U); break;
-        case 5: launch_vol_fast<5>(c, U); break;
+        case 0: launch_vol_fast<0>(c, U, ext); break;
+        case 1: launch_vol_fast<1>(c, U, ext); break;
+        case 2: launch_vol_fast<2>(c, U, ext); break;
+        case 3: launch_vol_fast<3>(c, U, ext); break;
+        case 4: launch_vol_fast<4>(c, U, ext); break;
+        case 5: launch_vol_fast<5>(c, U, ext); break;
     }
 }
 
@@ -393,28 +401,38 @@ int launch_residual(fvdg_ctx* c, const double* U, int parts, double* ms_vol, dou
 // ---------------------------------------------------------------- limiter
 
 // lim_ext <- per-element extremes of Uprev (constructMinMax)
+template <int P>
+void launch_limit_extrema_t(fvdg_ctx* c, const double* Uprev, int only_avg) {
+    const dim3 grid(nblocks(c->E, fvdg_lim::LimShape<P>::ET));
+    hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<P>, grid, dim3(256), 0, c->stream, Uprev, c->tV.p, c->E, only_avg,
+                       c->lim_ext.p);
+}
 void launch_limit_extrema(fvdg_ctx* c, const double* Uprev, int only_avg) {
-    const int N = c->tab.N, Q = c->tab.Q, L = limiter_lanes(Q), EB = 256 / L;
-    const size_t lds = sizeof(double) * ((size_t)EB * 5 * N + 256);
-    const dim3 grid(nblocks(c->E, EB));
-    switch (L) {
-        case 16: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<16>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
-        case 32: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<32>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
-        case 64: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<64>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
-        default: hipLaunchKernelGGL(fvdg_lim::k_limit_extrema<128>, grid, dim3(256), lds, c->stream, Uprev, c->tV.p, c->E, N, Q, only_avg, c->lim_ext.p); break;
+    switch (c->order) {
+        case 0: launch_limit_extrema_t<0>(c, Uprev, only_avg); break;
+        case 1: launch_limit_extrema_t<1>(c, Uprev, only_avg); break;
+        case 2: launch_limit_extrema_t<2>(c, Uprev, only_avg); break;
+        case 3: launch_limit_extrema_t<3>(c, Uprev, only_avg); break;
+        case 4: launch_limit_extrema_t<4>(c, Uprev, only_avg); break;
+        default: launch_limit_extrema_t<5>(c, Uprev, only_avg); break;
     }
 }
 
 // U <- limited U (apply_1 when parts & 1 with the bounds of lim_ext, apply_2 when parts & 2)
+template <int P>
+void launch_limit_apply_t(fvdg_ctx* c, double* U, int parts) {
+    const dim3 grid(nblocks(c->E, fvdg_lim::LimShape<P>::ET));
+    hipLaunchKernelGGL(fvdg_lim::k_limit_apply<P>, grid, dim3(256), 0, c->stream, U, c->lim_ext.p, c->cfaces.p,
+                       c->fcell.p, c->tV.p, c->E, c->gamma, parts);
+}
 void launch_limit_apply(fvdg_ctx* c, double* U, int parts) {
-    const int N = c->tab.N, Q = c->tab.Q, L = limiter_lanes(Q), EB = 256 / L;
-    const size_t lds = sizeof(double) * ((size_t)EB * 5 * N + (size_t)EB * 10 + 256);
-    const dim3 grid(nblocks(c->E, EB));
-    switch (L) {
-        case 16: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<16>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
-        case 32: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<32>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
-        case 64: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<64>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
-        default: hipLaunchKernelGGL(fvdg_lim::k_limit_apply<128>, grid, dim3(256), lds, c->stream, U, c->lim_ext.p, c->cfaces.p, c->fcell.p, c->tV.p, c->E, N, Q, c->gamma, parts); break;
+    switch (c->order) {
+        case 0: launch_limit_apply_t<0>(c, U, parts); break;
+        case 1: launch_limit_apply_t<1>(c, U, parts); break;
+        case 2: launch_limit_apply_t<2>(c, U, parts); break;
+        case 3: launch_limit_apply_t<3>(c, U, parts); break;
+        case 4: launch_limit_apply_t<4>(c, U, parts); break;
+        default: launch_limit_apply_t<5>(c, U, parts); break;
     }
 }
 
@@ -429,6 +447,8 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
     static const double B[3] = {1.0, 0.25, 2.0 / 3.0};
     const double* src[3] = {c->U.p, c->Ua.p, c->Ub.p};
     double* dst[3] = {c->Ua.p, c->Ub.p, c->U.p};
+    // point bounds of the stage input come out of the FP64 volume kernel itself
+    const bool fused_extrema = (c->lim_parts & FVDG_LIMIT_BOUNDS) && !c->lim_avg && !c->simple && !mixed(c);
     {
         if (c->simple) {
             int rc = launch_residual(c, src[s], FVDG_PART_ALL, ms ? &ms[1] : nullptr, ms ? &ms[2] : nullptr);
@@ -441,13 +461,13 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
             su.stop();
         } else {
             Span sv(ms ? &ms[1] : nullptr, c->stream);
-            dispatch_vol(c, src[s]);
+            dispatch_vol(c, src[s], fused_extrema ? c->lim_ext.p : nullptr);
             sv.stop();
             const StageUpdate up{c->U.p, dst[s], A[s], B[s], dt};
             dispatch_faces(c, src[s], FVDG_PART_ALL, true, &up, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
         }
         if (c->lim_parts) {  // limit the stage result, bounds from the stage input
-            if (c->lim_parts & FVDG_LIMIT_BOUNDS) launch_limit_extrema(c, src[s], c->lim_avg);
+            if ((c->lim_parts & FVDG_LIMIT_BOUNDS) && !fused_extrema) launch_limit_extrema(c, src[s], c->lim_avg);
             launch_limit_apply(c, dst[s], c->lim_parts);
             c->lim_bounds = (c->lim_parts & FVDG_LIMIT_BOUNDS) != 0;
         }
