@@ -94,6 +94,15 @@ def lib():
         _sig(L, "fvdg_conservation_sums", [_vp, _d])
         _sig(L, "fvdg_time_stages", [_vp, C.c_double, C.c_int, _d])
         _sig(L, "fvdg_flow_diagnostics", [_vp, _d])
+        _ip = C.POINTER(C.c_int)
+        _sig(L, "fvdg_implicit_assemble", [_vp, C.c_int])
+        _sig(L, "fvdg_implicit_blocks", [_vp, _d, _d, C.POINTER(C.c_int32), _d])
+        _sig(L, "fvdg_implicit_matvec", [_vp, C.c_double, _d, _d])
+        _sig(L, "fvdg_implicit_solve", [_vp, C.c_double, _d, _d, C.c_double, C.c_int, C.c_int, _ip, _d])
+        _sig(L, "fvdg_implicit_euler_step", [_vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _ip,
+                                             _ip, _d])
+        _sig(L, "fvdg_bsr_solve", [C.c_int, C.c_int64, C.c_int, _i64, _i64, _d, _d, _d, C.c_double, C.c_int, C.c_int,
+                                   _ip, _d])
         _sig(L, "fvdg_limiter_bounds", [_vp, _d, C.c_int])
         _sig(L, "fvdg_limiter_apply", [_vp, _d, C.c_int])
         _sig(L, "fvdg_set_stage_limiter", [_vp, C.c_int, C.c_int])
@@ -375,6 +384,44 @@ class Solver:
         _check(lib().fvdg_time_stages(self._h, dt, n_stages, _p(ms)))
         return ms
 
+    # ---- implicit path (ImplicitConvection::assemble, GMRES, backward-Euler Picard step)
+    def implicit_assemble(self, parts=PART_ALL):
+        """Picard blocks linearised at the device state (kept on the device)"""
+        _check(lib().fvdg_implicit_assemble(self._h, parts))
+        return self
+
+    def implicit_blocks(self):
+        """host copies: diag (E, NB, NB), off (E, 4, NB, NB), cols (E, 4), rhs (E, NB)"""
+        E, NB = self.n_cells, self.nv
+        out = dict(diag=np.zeros((E, NB, NB)), off=np.zeros((E, 4, NB, NB)), cols=np.zeros((E, 4), np.int32),
+                   rhs=np.zeros((E, NB)))
+        _check(lib().fvdg_implicit_blocks(self._h, _p(out["diag"]), _p(out["off"]), _p(out["cols"], C.c_int32),
+                                          _p(out["rhs"])))
+        return out
+
+    def implicit_matvec(self, x, shift=0.0):
+        """(A + shift M) x"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().fvdg_implicit_matvec(self._h, shift, _p(x), _p(y)))
+        return y
+
+    def implicit_solve(self, b=None, x0=None, shift=0.0, tol=1e-12, restart=30, max_iters=1000):
+        """solve (A + shift M) x = b (None: the assembled rhs); returns (x, iterations, relative residual)"""
+        x = np.zeros((self.n_cells, self.nv)) if x0 is None else np.array(x0, dtype=np.float64, order="C")
+        b_ = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        it, rr = C.c_int(), C.c_double()
+        _check(lib().fvdg_implicit_solve(self._h, shift, _p(b_), _p(x), tol, restart, max_iters, C.byref(it),
+                                         C.byref(rr)))
+        return x, it.value, rr.value
+
+    def implicit_euler_step(self, dt, max_picard=100, picard_tol=1e-8, tol=1e-10, restart=30, max_iters=1000):
+        """one backward-Euler Picard step of the device state; returns the iteration counts"""
+        pi, li, d = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().fvdg_implicit_euler_step(self._h, dt, max_picard, picard_tol, tol, restart, max_iters,
+                                              C.byref(pi), C.byref(li), C.byref(d)))
+        return dict(picard_iters=pi.value, linear_iters=li.value, delta=d.value)
+
     DIAGNOSTICS = ("volume", "mass", "kinetic_energy", "enstrophy", "total_energy", "min_rho", "min_p", "max_mach")
 
     def flow_diagnostics(self):
@@ -409,6 +456,22 @@ class Solver:
         """limit every SSP-RK3 stage result (parts 0: off)"""
         _check(lib().fvdg_set_stage_limiter(self._h, parts, int(bool(neighbour_averages))))
         return self
+
+
+def bsr_solve(row_ptr, cols, blocks, b, x0=None, tol=1e-12, restart=30, max_iters=1000, device=0):
+    """GMRES solve of a block-sparse system (CSR of square blocks, row-major) on the GPU;
+    returns (x, iterations, relative residual)"""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    blocks = np.ascontiguousarray(blocks, dtype=np.float64)
+    nb = blocks.shape[1]
+    n_rows = len(row_ptr) - 1
+    b = np.ascontiguousarray(b, dtype=np.float64).reshape(n_rows * nb)
+    x = np.zeros(n_rows * nb) if x0 is None else np.array(x0, dtype=np.float64).reshape(n_rows * nb)
+    it, rr = C.c_int(), C.c_double()
+    _check(lib().fvdg_bsr_solve(device, n_rows, nb, _p(row_ptr, C.c_int64), _p(cols, C.c_int64), _p(blocks), _p(b),
+                                _p(x), tol, restart, max_iters, C.byref(it), C.byref(rr)))
+    return x, it.value, rr.value
 
 
 class Ensemble:

@@ -13,7 +13,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfvdg.so"
 
-SOURCES = ["fvdg_host.cpp", "fvdg_solver.hip", "fvdg_ensemble.hip"]
+SOURCES = ["fvdg_host.cpp", "fvdg_solver.hip", "fvdg_ensemble.hip", "fvdg_implicit.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

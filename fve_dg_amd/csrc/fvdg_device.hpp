@@ -99,6 +99,11 @@ struct fvdg_ctx {
     bool lim_bounds = false;         // lim_ext holds bounds of the current mesh/scheme
     int lim_parts = 0, lim_avg = 0;
     DevBuf<double> diag;             // flow diagnostics: 8 partials per block
+    // implicit path (allocated on first use): assembled block rows (5 slots), right-hand side,
+    // per-unknown mass, block-Jacobi factors, GMRES and Picard work vectors
+    DevBuf<double> imp_A, imp_rhs, imp_b, imp_Md, imp_LU, imp_V, imp_w, imp_z, imp_part, imp_xold, imp_dx;
+    DevBuf<int32_t> imp_col, imp_piv, imp_dslot, imp_flag;
+    bool imp_have = false;           // imp_A holds a system assembled on the current mesh and scheme
 
     ~fvdg_ctx() {
         if (stream && owns_stream) (void)hipStreamDestroy(stream);
@@ -111,4 +116,7 @@ namespace fvdg {
 int ssprk3_once(fvdg_ctx* c, double dt, double* ms);
 // per face: max over its points of the traces' |u| + c (host copy of F values)
 int face_wave_speeds(fvdg_ctx* c, std::vector<double>& out);
+// PositiveLimiter on device arrays: bounds from prev (constructMinMax; skipped when parts has
+// no FVDG_LIMIT_BOUNDS), then apply to cur in place (stream-ordered)
+void limit_state(fvdg_ctx* c, const double* prev, double* cur, int parts, int neighbour_averages);
 }  // namespace fvdg

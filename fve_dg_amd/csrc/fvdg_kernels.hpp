@@ -11,6 +11,8 @@
 // thread-private loop, so each output has one fixed summation order.  They serve
 // as the GPU cross-check of the tuned kernels (fvdg_kernels_fast.hpp) at sizes the
 // CPU oracle cannot reach.
+// FVDG_DEVICE_HELPERS_ONLY (defined by other translation units): the device helpers without the
+// non-template kernels, which are defined in the solver's translation unit.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -183,6 +185,7 @@ __device__ __forceinline__ void num_flux(int kind, const double UL[5], const dou
 
 // ------------------------------------------------------------------ volume term
 // one thread per (element, volume point): trace of U and its physical flux
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_vol_flux(KernelArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.E * a.Q) return;
@@ -200,8 +203,10 @@ __global__ void k_vol_flux(KernelArgs a) {
     for (int v = 0; v < 5; ++v)
         for (int d = 0; d < 3; ++d) out[v * 3 + d] = F[v][d];
 }
+#endif
 
 // one thread per (element, test function j): R = -sum_q (F . A^{-1} grad phi_j) w_q |J|
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_vol_test(KernelArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.E * a.N) return;
@@ -223,6 +228,7 @@ __global__ void k_vol_test(KernelArgs a) {
     }
     for (int v = 0; v < 5; ++v) a.R[cidx(e, 5 * j + v, NV)] = r[v];
 }
+#endif
 
 // ------------------------------------------------------------------ face term
 __device__ __forceinline__ void face_trace(const KernelArgs& a, int64_t cell, int combo, int g, double U[5]) {
@@ -251,6 +257,7 @@ __device__ __forceinline__ bool ghost_state(const KernelArgs& a, int64_t f, int 
 }
 
 // one thread per (face, face point): the numerical flux Fhat(U_L, U_R, n)
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_face_flux(KernelArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.F * a.Qf) return;
@@ -268,8 +275,10 @@ __global__ void k_face_flux(KernelArgs a) {
     if (active) num_flux(a.flux, UL, UR, n, a.gamma, out);
     for (int v = 0; v < 5; ++v) a.Fh[t * 5 + v] = out[v];
 }
+#endif
 
 // one thread per (element, test function j): add the four faces' contributions
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_face_test(KernelArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.E * a.N) return;
@@ -302,9 +311,11 @@ __global__ void k_face_test(KernelArgs a) {
         a.R[i] = acc ? a.R[i] + r[v] : r[v];
     }
 }
+#endif
 
 // ------------------------------------------------------------------ SSP-RK stage update
 // out = a0*U0 + b0*(Us + dt * (-R / M)),  M = |det A_e| * mass_hat[k]
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_rk_update(const double* __restrict__ U0, const double* __restrict__ Us,
                             const double* __restrict__ R, double* __restrict__ out, const double* __restrict__ cgeo,
                             const double* __restrict__ mhat, int64_t E, int N, double a0, double b0, double dt,
@@ -319,8 +330,10 @@ __global__ void k_rk_update(const double* __restrict__ U0, const double* __restr
     const double s = Us[i] + (edt ? edt[e] : dt) * L;
     out[i] = (a0 == 0.0) ? b0 * s : a0 * U0[i] + b0 * s;
 }
+#endif
 
 // ------------------------------------------------------------------ reductions
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_max_wave_speed(KernelArgs a, double* __restrict__ part) {
     __shared__ double sm[256];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -344,8 +357,10 @@ __global__ void k_max_wave_speed(KernelArgs a, double* __restrict__ part) {
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
+#endif
 
 // per face: max over its points of max(|u|+c) of the traces (one thread per face)
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_face_wave_speed(KernelArgs a, double* __restrict__ out) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= a.F) return;
@@ -361,8 +376,10 @@ __global__ void k_face_wave_speed(KernelArgs a, double* __restrict__ out) {
     }
     out[f] = m;
 }
+#endif
 
 // integral of each conserved variable: |det A_e| * c_{e,0,v} * (1/6), phi_0 = 1
+#ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_conservation(const double* __restrict__ U, const double* __restrict__ cgeo, int64_t E, int N,
                                double* __restrict__ part) {
     __shared__ double sm[5][256];
@@ -378,3 +395,4 @@ __global__ void k_conservation(const double* __restrict__ U, const double* __res
     if (threadIdx.x == 0)
         for (int v = 0; v < 5; ++v) part[blockIdx.x * 5 + v] = sm[v][0];
 }
+#endif

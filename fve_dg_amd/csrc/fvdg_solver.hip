@@ -135,6 +135,11 @@ int alloc_state(fvdg_ctx* c) {
     HIP_TRY(c->diag.alloc((size_t)point_blocks(c) * fvdg_diag::kNumDiag));
     c->lim_buf.release();
     c->lim_bounds = false;
+    for (DevBuf<double>* b : {&c->imp_A, &c->imp_rhs, &c->imp_b, &c->imp_Md, &c->imp_LU, &c->imp_V, &c->imp_w,
+                              &c->imp_z, &c->imp_part, &c->imp_xold, &c->imp_dx})
+        b->release();
+    for (DevBuf<int32_t>* b : {&c->imp_col, &c->imp_piv, &c->imp_dslot, &c->imp_flag}) b->release();
+    c->imp_have = false;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->have_state = false;
     return FVDG_OK;
@@ -487,6 +492,11 @@ int fvdg::ssprk3_once(fvdg_ctx* c, double dt, double* ms) {
     return FVDG_OK;
 }
 
+void fvdg::limit_state(fvdg_ctx* c, const double* prev, double* cur, int parts, int neighbour_averages) {
+    if (parts & FVDG_LIMIT_BOUNDS) launch_limit_extrema(c, prev, neighbour_averages);
+    launch_limit_apply(c, cur, parts);
+}
+
 int fvdg::face_wave_speeds(fvdg_ctx* c, std::vector<double>& out) {
     int rc = check_ready(c, true);
     if (rc) return rc;
@@ -595,7 +605,8 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
         fgeo[4 * f + 3] = Fc.compute_jacobian_det();
         const BoundaryType bt = f < (int64_t)cm.m_boundaryTypes.size() ? cm.m_boundaryTypes[f] : BoundaryType::Dirichlet;
         fbc[f] = bnd ? (uint8_t)bt : (uint8_t)255;
-        if (bnd && bt == BoundaryType::Dirichlet) {
+        // faces that take boundary data: Dirichlet (both paths), WallTD / WallTN (implicit path)
+        if (bnd && (bt == BoundaryType::Dirichlet || bt == BoundaryType::WallTD || bt == BoundaryType::WallTN)) {
             fdir[f] = (int32_t)dir_faces.size();
             dir_faces.push_back(f);
         }

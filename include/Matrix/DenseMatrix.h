@@ -43,6 +43,13 @@ public:
         for (uInt i = 0; i < std::min(M, N); ++i) r(i, i) = 1.0;
         return r;
     }
+    // square matrix with d on the diagonal (reference use: examples/Eular2D/Sedov/Sedov.cpp:164)
+    static DenseMatrix Diag(const DenseMatrix<M, 1>& d) {
+        static_assert(M == N, "Diag builds a square matrix");
+        DenseMatrix r(0.0);
+        for (uInt i = 0; i < M; ++i) r(i, i) = d[i];
+        return r;
+    }
 
     Scalar& operator()(uInt r, uInt c) { return data[r * N + c]; }
     const Scalar& operator()(uInt r, uInt c) const { return data[r * N + c]; }

@@ -1,7 +1,6 @@
-// Umbrella header (reference: include/Matrix/Matrix.h).  The sparse/Krylov
-// part of the reference's Matrix library serves its implicit solver and is
-// not provided here.
+// Umbrella header (reference: include/Matrix/Matrix.h).
 #pragma once
 #include "base/Type.h"
 #include "Matrix/DenseMatrix.h"
 #include "Matrix/LongVector.h"
+#include "Matrix/SparseMatrix.h"

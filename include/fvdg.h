@@ -140,8 +140,9 @@ int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precisio
 int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m);
 int fvdg_state_upload(fvdg_ctx* c, const double* U);
 int fvdg_state_download(fvdg_ctx* c, double* U);
-/* Dirichlet data: number of Dirichlet boundary faces, and for each its face id and the
- * physical positions of its face quadrature points (xyz[(i*Qf+g)*3+d]); either pointer may be NULL. */
+/* Boundary data: number of boundary faces that take data (Dirichlet; WallTD and WallTN, used by
+ * the implicit path only), and for each its face id and the physical positions of its face
+ * quadrature points (xyz[(i*Qf+g)*3+d]); either pointer may be NULL. */
 int fvdg_dirichlet_points(fvdg_ctx* c, int64_t* n_faces, int64_t* face_ids, double* xyz);
 /* States U_D at those points, UD[(i*Qf+g)*5+v]; used by every later residual. */
 int fvdg_set_dirichlet_states(fvdg_ctx* c, const double* UD);
@@ -169,6 +170,43 @@ int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);
  * kernels gives ms[1] = volume, ms[2] = face flux, ms[3] = face test + stage update
  * (per-stage averages). */
 int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms);
+
+/* ---------------- implicit path: ImplicitConvection<Order,Flux>::assemble
+ * (include/DG/DG_Schemes/ImplicitConvection.h:10-54, src/DG/DG_Schemes/ImplicitConvection.cpp)
+ * The Picard-linearised system R(U) ~ A(U_lin) U - rhs at the device state U_lin: per element a
+ * diagonal block and one block per interior face (NB x NB, NB = 5N, row 5j+v, column 5i+u),
+ * boundary faces linearised around the reference's ghost states (Dirichlet / WallTD / WallTN
+ * with the data of fvdg_set_dirichlet_states, Pseudo3DX/Y/Z mirrors).  Kept on the device. */
+int fvdg_implicit_assemble(fvdg_ctx* c, int parts);
+/* host copies of the last assembly (any pointer may be NULL): diag[E][NB][NB] and off[E][4][NB][NB]
+ * row-major, off_cols[E][4] the neighbour across local face l (-1: boundary face, zero block),
+ * rhs[E][NB] */
+int fvdg_implicit_blocks(fvdg_ctx* c, double* diag, double* off, int32_t* off_cols, double* rhs);
+/* y = (A + shift M) x, M the diagonal mass (host vectors, n_cells x 5N) */
+int fvdg_implicit_matvec(fvdg_ctx* c, double shift, const double* x, double* y);
+/* solve (A + shift M) x = b (b NULL: the assembled rhs) by restarted GMRES(restart) with a
+ * block-Jacobi preconditioner (LU of each diagonal block + shift M), from the guess in x, to a
+ * relative residual |b - (A + shift M) x| / |b| < tol or max_iters iterations; host vectors.
+ * iters / rel_res (nullable): iterations done and the final true relative residual.
+ * Returns FVDG_ERR_NONFINITE when a diagonal block is singular. */
+int fvdg_implicit_solve(fvdg_ctx* c, double shift, const double* b, double* x, double tol, int restart,
+                        int max_iters, int* iters, double* rel_res);
+/* One backward-Euler step of the device state by Picard iteration, as the reference's implicit
+ * drivers do (examples/Eular2D/Sedov/Sedov.cpp:137-226): dx = U; repeat { assemble at dx; solve
+ * (M/dt + A) x = M/dt U + rhs from dx; limit x against dx if a stage limiter is set;
+ * delta = |x - dx|; dx = x } until delta < picard_tol, delta / previous delta < picard_tol,
+ * after 10 iterations delta / first delta > 0.95, or max_picard iterations; U = dx.
+ * picard_iters / linear_iters / delta (nullable): iterations done, GMRES iterations in total,
+ * last delta. */
+int fvdg_implicit_euler_step(fvdg_ctx* c, double dt, int max_picard, double picard_tol, double tol, int restart,
+                             int max_iters, int* picard_iters, int* linear_iters, double* delta);
+/* Solve a general block-sparse system (the reference's BlockSparseMatrix + EigenSparseSolver::DGMRES,
+ * include/Matrix/SparseMatrix.h, src/EigenSolver/EigenSparseSolver.cpp:163-195) on device `device`:
+ * n_rows block rows of size block, CSR row_ptr[n_rows+1] / cols[nnz] / blocks[nnz][block][block]
+ * row-major; same GMRES and stopping rule as fvdg_implicit_solve (shift 0). */
+int fvdg_bsr_solve(int device, int64_t n_rows, int block, const int64_t* row_ptr, const int64_t* cols,
+                   const double* blocks, const double* b, double* x, double tol, int restart, int max_iters,
+                   int* iters, double* rel_res);
 
 /* flow diagnostics of the device state, integrals at the volume quadrature points with the
  * weights w_q |det A| of the reference's err_integral (examples/Vortex/VortexBatch3.cpp:349-366):

@@ -55,6 +55,9 @@ class Oracle:
         _sig(L, "or_vortex_state", [_d, C.c_double, _d, _d], None)
         _sig(L, "or_project_vortex", [C.c_int, _d, C.c_double, C.c_int64, _d, C.c_int64, _i64, _d])
         _sig(L, "or_max_threads", [])
+        _sig(L, "or_jacobian", [_d, C.c_double, _d], None)
+        _sig(L, "or_implicit_assemble", [C.c_int, C.c_double, C.c_int64, _d, C.c_int64, _i64, C.c_int64, _i64, _i64,
+                                         _i64, _u8, _d, _d, _d, _d, _d])
         _sig(L, "or_limiter_bounds", [C.c_int, C.c_int64, C.c_int64, _i64, _d, C.c_int, _d, _d])
         _sig(L, "or_limiter_apply", [C.c_int, C.c_int64, _d, _d, C.c_double, C.c_int, _d])
         self.L = L
@@ -155,6 +158,31 @@ class Oracle:
 
     def max_threads(self):
         return self.L.or_max_threads()
+
+    # -------------------------------------------------- implicit (ImplicitConvection.cpp)
+    def jacobian(self, U, gamma=1.4):
+        """(3, 5, 5): dF_d/dU"""
+        J = np.zeros((3, 25))
+        self.L.or_jacobian(_p(np.ascontiguousarray(U, float)), gamma, _p(J))
+        return J.reshape(3, 5, 5)
+
+    def implicit_assemble(self, m, order, U, gamma=1.4, bc=None, UD=None):
+        """Picard blocks at U: diag (E, NB, NB), off (F, 2, NB, NB) = blocks (L, R) and (R, L) of
+        each interior face, rhs (E, NB); UD per Dirichlet / WallTD / WallTN face, (faces*Qf, 5)"""
+        a = _args(m)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        E, NB = U.shape
+        nf = len(a["face_nodes"])
+        diag, off, rhs = np.zeros((E, NB, NB)), np.zeros((nf, 2, NB, NB)), np.zeros((E, NB))
+        bc_ = None if bc is None else np.ascontiguousarray(bc, dtype=np.uint8)
+        UD_ = None if UD is None else np.ascontiguousarray(UD, dtype=np.float64)
+        rc = self.L.or_implicit_assemble(order, gamma, len(a["points"]), _p(a["points"]), E, _p(a["cells"], C.c_int64),
+                                         nf, _p(a["face_nodes"], C.c_int64), _p(a["face_cells"], C.c_int64),
+                                         _p(a["face_nodes_r"], C.c_int64), _p(bc_, C.c_uint8), _p(UD_), _p(U),
+                                         _p(diag), _p(off), _p(rhs))
+        if rc:
+            raise RuntimeError(f"or_implicit_assemble failed ({rc})")
+        return diag, off, rhs
 
     # -------------------------------------------------- limiter (PositiveLimiter.h)
     def limiter_bounds(self, m, order, U_prev, only_avg=False):

@@ -25,6 +25,10 @@
  *                      examples/Vortex/123/VortexBatch.cpp:189-197 generalised to 3 stages)
  *   or_vortex_state, or_project
  *                      examples/Vortex/problem.h:17-110, include/DG/DG_Basis/DG_Basis_Evaluator.h:33-59
+ *   or_jacobian        src/DG/DG_Flux/EulerPhysicalFlux.cpp:21-42
+ *   or_implicit_assemble
+ *                      src/DG/DG_Schemes/ImplicitConvection.cpp:20-167 (boundarys), :266-325 (cells),
+ *                      :329-420 (internals): the Picard blocks and right-hand side
  *   or_limiter_bounds, or_limiter_apply
  *                      include/DG/DG_Schemes/PositiveLimiter.h:62-119 (constructMinMax), :135-170
  *                      (apply_1), :171-198 (apply_21), :199-259 (apply_22); that header cannot be
@@ -719,6 +723,196 @@ int or_limiter_apply(int order, int64_t nc, const double* cmax, const double* cm
                 for (int l = 1; l < N; ++l) c[k + 5 * l] *= thp;
         }
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ implicit (Picard) blocks */
+
+/* dF_d/dU, d = x, y, z (EulerPhysicalFlux.cpp:21-42): J[d][5 r + c] */
+void or_jacobian(const double U[5], double gamma, double J[3][25]) {
+    const double u = U[1] / U[0], v = U[2] / U[0], w = U[3] / U[0], e = U[4] / U[0];
+    const double q2 = u * u + v * v + w * w, g1 = gamma - 1.0;
+    const double vel[3] = {u, v, w};
+    for (int d = 0; d < 3; ++d) {
+        double* A = J[d];
+        for (int i = 0; i < 25; ++i) A[i] = 0.0;
+        const double a = vel[d];
+        A[1 + d] = 1.0;  /* mass row: the momentum component of direction d */
+        for (int m = 0; m < 3; ++m) {  /* momentum rows 1..3 */
+            double* row = A + 5 * (1 + m);
+            row[0] = -vel[m] * a + (m == d ? 0.5 * g1 * q2 : 0.0);
+            row[1 + m] += a;
+            row[1 + d] += vel[m];
+            if (m == d)
+                for (int k = 0; k < 3; ++k) row[1 + k] -= g1 * vel[k];
+            if (m == d) row[4] = g1;
+        }
+        double* er = A + 20;  /* energy row */
+        er[0] = a * (-gamma * e + g1 * q2);
+        for (int k = 0; k < 3; ++k) er[1 + k] = -g1 * a * vel[k];
+        er[1 + d] = gamma * e - 0.5 * g1 * (2.0 * a * a + q2);
+        er[4] = gamma * a;
+    }
+}
+
+static void normal_jacobian(const double J[3][25], const double n[3], double Fn[25]) {
+    for (int i = 0; i < 25; ++i) Fn[i] = J[0][i] * n[0] + J[1][i] * n[1] + J[2][i] * n[2];
+}
+
+/* ghost-state linearisation of one boundary point (ImplicitConvection.cpp:20-167):
+ * dF (5x5) multiplies the interior trace, drhs is the explicit remainder */
+static void implicit_boundary(int bc, const double Uin[5], const double* UD, const double n[3], double gamma,
+                              double dF[25], double drhs[5]) {
+    double Ug[5], Jg[3][25], Fn[25];
+    for (int i = 0; i < 25; ++i) dF[i] = 0.0;
+    for (int v = 0; v < 5; ++v) drhs[v] = 0.0;
+    if (bc == 0 || bc == 9 || bc == 10) { /* Dirichlet, WallTD, WallTN */
+        for (int v = 0; v < 5; ++v) Ug[v] = UD[v] + (UD[v] - Uin[v]);
+        if (bc != 0) {
+            Ug[0] = Uin[0];
+            const double cvT_in = Uin[4] - 0.5 * (Uin[1] * Uin[1] + Uin[2] * Uin[2] + Uin[3] * Uin[3]) / Uin[0];
+            const double cvT_g = (bc == 9) ? 2 * 300 - cvT_in : cvT_in;
+            Ug[4] = cvT_g + 0.5 * (Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3]) / Ug[0];
+        }
+        for (int v = 0; v < 5; ++v) Ug[v] = 0.5 * (Uin[v] + Ug[v]);
+        or_jacobian(Ug, gamma, Jg);
+    } else if (bc == 3 || bc == 4 || bc == 5) { /* Pseudo3DX / Y / Z: mirror of that momentum */
+        for (int v = 0; v < 5; ++v) Ug[v] = Uin[v];
+        Ug[bc - 2] = -Uin[bc - 2];
+        double Um[5];
+        for (int v = 0; v < 5; ++v) Um[v] = 0.5 * (Uin[v] + Ug[v]);
+        or_jacobian(Um, gamma, Jg);
+    } else {
+        return;
+    }
+    normal_jacobian(Jg, n, Fn);
+    const double lam = or_wave_speed(Uin, Ug, gamma);
+    for (int v = 0; v < 5; ++v) Ug[v] = 0.5 * (Uin[v] + Ug[v]);
+    for (int r = 0; r < 5; ++r) {
+        double fu = 0.0;
+        for (int c = 0; c < 5; ++c) {
+            dF[5 * r + c] = 0.5 * (Fn[5 * r + c] + (r == c ? lam : 0.0));
+            fu += Fn[5 * r + c] * Ug[c];
+        }
+        drhs[r] = 0.5 * (fu - lam * Ug[r]);
+    }
+}
+
+/* ImplicitConvection<Order>::assemble (src/DG/DG_Schemes/ImplicitConvection.cpp:169-448): the Picard
+ * blocks of R(U) ~ A(U_lin) U - rhs, linearised at U, accumulated into (zeroed by the caller)
+ *   diag[e]    (NB x NB, NB = 5N, row 5j+v = test function j / variable v, column 5i+u),
+ *   off[2f]    block (L, R) of interior face f, off[2f+1] block (R, L),
+ *   rhs[e]     (NB).
+ * Boundary data UD: per face of type Dirichlet / WallTD / WallTN in face order, Qf x 5 each. */
+int or_implicit_assemble(int order, double gamma, int64_t np, const double* pts, int64_t nc, const int64_t* cells,
+                         int64_t nf, const int64_t* fnodes, const int64_t* fcells, const int64_t* fnodes_r,
+                         const uint8_t* bc, const double* UD, const double* U, double* diag, double* off,
+                         double* rhs) {
+    scheme_t S;
+    S.order = order;
+    S.N = or_num_basis(order);
+    S.Q = or_tet_rule(order, &S.qp, &S.qw);
+    S.Qf = or_tri_rule(order, &S.fp, &S.fw);
+    if (!S.Q || !S.Qf) return -1;
+    const int N = S.N, NB = 5 * N;
+    const size_t BB = (size_t)NB * NB;
+    double* cA = (double*)malloc(sizeof(double) * 9 * nc);
+    double* cdet = (double*)malloc(sizeof(double) * nc);
+    double* fn = (double*)malloc(sizeof(double) * 3 * nf);
+    double* fa = (double*)malloc(sizeof(double) * nf);
+    int* fv = (int*)malloc(sizeof(int) * 6 * nf);
+    or_geometry(np, pts, nc, cells, nf, fnodes, fcells, fnodes_r, cA, cdet, fn, fa, fv);
+    double* vb = (double*)malloc(sizeof(double) * S.Q * N);
+    double* vg = (double*)malloc(sizeof(double) * S.Q * 3 * N);
+    for (int q = 0; q < S.Q; ++q) or_basis(order, S.qp[3 * q], S.qp[3 * q + 1], S.qp[3 * q + 2], vb + q * N, vg + q * 3 * N);
+    double phiL[64], phiR[64];
+
+    /* assemble_cells (:266-325): block (j, i) -= sum_q w |det| phi_i (sum_d J_d(U_q) d phi_j / d x_d) */
+    for (int64_t e = 0; e < nc; ++e) {
+        const double* coef = U + e * NB;
+        double Ai[9];
+        or_inverse3(cA + 9 * e, Ai);
+        for (int q = 0; q < S.Q; ++q) {
+            double Uq[5] = {0, 0, 0, 0, 0}, J[3][25];
+            for (int k = 0; k < N; ++k)
+                for (int v = 0; v < 5; ++v) Uq[v] += vb[q * N + k] * coef[5 * k + v];
+            or_jacobian(Uq, gamma, J);
+            const double jw = S.qw[q] * cdet[e];
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) {
+                    const double* gr = vg + (q * N + j) * 3;
+                    double gp[3];
+                    for (int d = 0; d < 3; ++d) gp[d] = Ai[3 * d] * gr[0] + Ai[3 * d + 1] * gr[1] + Ai[3 * d + 2] * gr[2];
+                    const double s = vb[q * N + i] * jw;
+                    for (int v = 0; v < 5; ++v)
+                        for (int u = 0; u < 5; ++u)
+                            diag[e * BB + (size_t)(5 * j + v) * NB + 5 * i + u] -=
+                                (J[0][5 * v + u] * gp[0] + J[1][5 * v + u] * gp[1] + J[2][5 * v + u] * gp[2]) * s;
+                }
+        }
+    }
+
+    /* assemble_internals (:329-420) and assemble_boundarys (:20-167) */
+    int64_t nd = 0;
+    for (int64_t f = 0; f < nf; ++f) {
+        const int64_t c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
+        const double jf = fa[f] * 2.0;
+        const double* n = fn + 3 * f;
+        int64_t slot = -1;
+        if (c1 < 0 && bc && (bc[f] == 0 || bc[f] == 9 || bc[f] == 10)) slot = nd++;
+        for (int g = 0; g < S.Qf; ++g) {
+            const double wgt = S.fw[g] * jf;
+            double xi[3], UL[5] = {0, 0, 0, 0, 0}, UR[5] = {0, 0, 0, 0, 0};
+            face_to_cell(fv + 6 * f, S.fp[2 * g], S.fp[2 * g + 1], xi);
+            or_basis(order, xi[0], xi[1], xi[2], phiL, NULL);
+            for (int k = 0; k < N; ++k)
+                for (int v = 0; v < 5; ++v) UL[v] += phiL[k] * U[c0 * NB + 5 * k + v];
+            if (c1 >= 0) {
+                face_to_cell(fv + 6 * f + 3, S.fp[2 * g], S.fp[2 * g + 1], xi);
+                or_basis(order, xi[0], xi[1], xi[2], phiR, NULL);
+                for (int k = 0; k < N; ++k)
+                    for (int v = 0; v < 5; ++v) UR[v] += phiR[k] * U[c1 * NB + 5 * k + v];
+                double JL[3][25], JR[3][25], FnL[25], FnR[25], dFL[25], dFR[25];
+                or_jacobian(UL, gamma, JL);
+                or_jacobian(UR, gamma, JR);
+                normal_jacobian(JL, n, FnL);
+                normal_jacobian(JR, n, FnR);
+                const double lam = or_wave_speed(UL, UR, gamma);
+                for (int r = 0; r < 5; ++r)
+                    for (int c = 0; c < 5; ++c) {
+                        dFL[5 * r + c] = 0.5 * (FnL[5 * r + c] + (r == c ? lam : 0.0));
+                        dFR[5 * r + c] = 0.5 * (FnR[5 * r + c] - (r == c ? lam : 0.0));
+                    }
+                for (int i = 0; i < N; ++i)
+                    for (int j = 0; j < N; ++j)
+                        for (int v = 0; v < 5; ++v)
+                            for (int u = 0; u < 5; ++u) {
+                                const size_t ix = (size_t)(5 * j + v) * NB + 5 * i + u;
+                                diag[c0 * BB + ix] += dFL[5 * v + u] * (phiL[i] * phiL[j]) * wgt;
+                                off[(2 * f) * BB + ix] += dFR[5 * v + u] * (phiR[i] * phiL[j]) * wgt;
+                                off[(2 * f + 1) * BB + ix] -= dFL[5 * v + u] * (phiL[i] * phiR[j]) * wgt;
+                                diag[c1 * BB + ix] -= dFR[5 * v + u] * (phiR[i] * phiR[j]) * wgt;
+                            }
+            } else if (bc) {
+                double dF[25], drhs[5];
+                implicit_boundary(bc[f], UL, slot >= 0 ? UD + (slot * S.Qf + g) * 5 : NULL, n, gamma, dF, drhs);
+                for (int i = 0; i < N; ++i)
+                    for (int j = 0; j < N; ++j)
+                        for (int v = 0; v < 5; ++v)
+                            for (int u = 0; u < 5; ++u)
+                                diag[c0 * BB + (size_t)(5 * j + v) * NB + 5 * i + u] += dF[5 * v + u] * (phiL[i] * phiL[j]) * wgt;
+                for (int j = 0; j < N; ++j)
+                    for (int v = 0; v < 5; ++v) rhs[c0 * NB + 5 * j + v] -= drhs[v] * phiL[j] * wgt;
+            }
+        }
+    }
+    free(cA);
+    free(cdet);
+    free(fn);
+    free(fa);
+    free(fv);
+    free(vb);
+    free(vg);
     return 0;
 }
 

@@ -44,3 +44,18 @@ def rel_err(a, b):
     """max |a-b| / max |b| (norm-wise relative deviation)"""
     a, b = np.asarray(a), np.asarray(b)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def project_pointwise(oracle, m, order, fn):
+    """diagonal L2 projection of a pointwise state fn(x) -> (5,) with the oracle's basis and rule
+    (exact for fields in the order's polynomial space)"""
+    pts, w = oracle.quadrature(order, 0)
+    og = oracle.geometry(m)
+    Phi = np.array([oracle.basis(order, *x)[0] for x in pts])  # Q x N
+    diag = (w[:, None] * Phi ** 2).sum(0)
+    U = np.zeros((len(m["cells"]), 5 * Phi.shape[1]))
+    for e, cell in enumerate(m["cells"]):
+        X = m["points"][cell[0]] + pts @ og["A"][e]  # x(xi) = p0 + A^T xi, rows of A = edges
+        vals = np.array([fn(x) for x in X])
+        U[e] = ((w[:, None, None] * Phi[:, :, None] * vals[:, None, :]).sum(0) / diag[:, None]).ravel()
+    return U

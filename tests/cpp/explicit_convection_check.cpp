@@ -3,7 +3,8 @@
 // example (examples/Vortex/tools.h:22-31), defines the exact-solution hooks of
 // base/exact.h, projects the vortex with DGBasisEvaluator::func2coef, and calls
 // ExplicitConvection<P>::eval / eval_cells / eval_internals / eval_boundarys, the
-// ExplicitSSPRK3 driver and PositiveLimiter (constructMinMax, apply, apply_1, apply_2).  Everything it computes is written to a binary file that
+// ExplicitSSPRK3 driver, PositiveLimiter (constructMinMax, apply, apply_1, apply_2), and one
+// backward-Euler solve with ImplicitConvection + BlockSparseMatrix + EigenSparseSolver.  Everything it computes is written to a binary file that
 // tests/test_cpp_dropin.py checks against the CPU oracle.
 //
 //   explicit_convection_check OUT.bin
@@ -14,7 +15,9 @@
 
 #include "DG/DG_Schemes/ExplicitConvection.h"
 #include "DG/DG_Schemes/ExplicitSSPRK3.h"
+#include "DG/DG_Schemes/ImplicitConvection.h"
 #include "DG/DG_Schemes/PositiveLimiter.h"
+#include "EigenSolver/EigenSparseSolver.h"
 
 // ---- the vortex of examples/Vortex/problem.h:7-110 (centre (4, 4.5), free stream (1, 0.5, 0), strength 5)
 namespace {
@@ -190,6 +193,49 @@ void limiter_case(Writer& w) {
     w.vec(B);
 }
 
+// one backward-Euler system as the reference's implicit drivers build it
+// (examples/Eular2D/Sedov/Sedov.cpp:153-180): assemble at U, add mass/dt blocks and mass/dt U to the
+// right-hand side, finalize, DGMRES from U
+template <uInt P>
+void implicit_case(Writer& w) {
+    constexpr uInt NB = 5 * DGBasisEvaluator<P>::NumBasis;
+    OrthHexMesh g({0, 0, 0}, {10, 10, 10}, {3, 3, 3});
+    g.split_hex6_scan();
+    g.make_periodic({0, 0, 0}, {10, 10, 10}, {true, true, true});
+    g.jitter_points(0.1, 11, {0, 0, 0}, {10, 10, 10}, {10.0 / 3, 10.0 / 3, 10.0 / 3}, {true, true, true});
+    ComputingMesh cm(g);
+    const auto U = project<P>(cm, 0.0);
+    const Scalar dt = 0.05;
+    ExplicitSSPRK3<P> rk(cm);
+    const auto mass = rk.mass();
+    ImplicitConvection<P> convection;
+    BlockSparseMatrix<NB, NB> sparse_mat;
+    LongVector<NB> rhs(U.size());
+    convection.assemble(cm, U, 0.0, sparse_mat, rhs);
+    for (uInt e = 0; e < cm.m_cells.size(); ++e) {
+        sparse_mat.add_block(e, e, DenseMatrix<NB, NB>::Diag(mass[e] / dt));
+        rhs[e] += mass[e] / dt * U[e];
+    }
+    sparse_mat.finalize();
+    EigenSparseSolver<NB, NB> solver(sparse_mat, rhs);
+    solver.set_tol(1e-13);
+    uInt iter = 0;
+    Scalar residual = 0;
+    const auto x = solver.DGMRES(U, iter, residual);
+    const auto Ax = sparse_mat.multiply(x);
+    const int64_t it = iter;
+    w.i64(P);
+    w.mesh(cm);
+    w.vec(U);
+    w.f64(&dt, 1);
+    w.vec(mass);
+    w.vec(rhs);
+    w.vec(x);
+    w.vec(Ax);
+    w.i64(it);
+    w.f64(&residual, 1);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -199,11 +245,12 @@ int main(int argc, char** argv) {
     }
     try {
         Writer w(argv[1]);
-        w.i64(4);  // number of cases
+        w.i64(5);  // number of cases
         boundary_case<1>(w, FVDG_FLUX_LF);
         boundary_case<3>(w, FVDG_FLUX_ROE);
         periodic_case<2>(w, FVDG_FLUX_ROE);
         limiter_case<2>(w);
+        implicit_case<1>(w);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "explicit_convection_check: %s\n", e.what());
         return 2;

@@ -10,7 +10,10 @@ runs on the GPU and writes its inputs and results; they are checked here against
   Shu-Osher composition of oracle residuals at those times;
 * a periodic jittered mesh with gamma = 5/3 (MonatomicFluxC) and the Roe flux;
 * PositiveLimiter<2> (constructMinMax + apply) and PositiveLimiter<2, QuadC, true>
-  (constructMinMax + apply_1 + apply_2) on a sharpened vortex equal the oracle's limiter.
+  (constructMinMax + apply_1 + apply_2) on a sharpened vortex equal the oracle's limiter;
+* one backward-Euler system built as the reference's implicit drivers build it
+  (ImplicitConvection<1>::assemble + mass/dt blocks + finalize + EigenSparseSolver::DGMRES) equals
+  the dense solve of the oracle's assembly, and BlockSparseMatrix::multiply equals its product.
 """
 import subprocess
 from pathlib import Path
@@ -105,7 +108,7 @@ def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     r = subprocess.run([str(EXE), str(out)], capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stderr
     rd = Reader(out)
-    assert rd.i64() == 4
+    assert rd.i64() == 5
     # ---- Dirichlet / Pseudo3DZ boxes, P1 LF and P3 Roe
     for _ in range(2):
         order, flux_id = rd.i64(), rd.i64()
@@ -166,4 +169,26 @@ def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     expB = oracle.limiter_apply(order, oracle.limiter_apply(order, U, bB, parts=1), bB, parts=2)
     assert rel_err(A, U) > 1e-3 and rel_err(B, U) > 1e-3  # the limiter acted
     assert rel_err(A, expA) < 1e-12 and rel_err(B, expB) < 1e-12
+    # ---- implicit system, P1
+    order = rd.i64()
+    m, _ = rd.mesh()
+    E, N = len(m["cells"]), F.num_basis(order)
+    NB = 5 * N
+    U = rd.f64(E * NB).reshape(E, NB)
+    dt = rd.f64(1)[0]
+    M, rhs, x, Ax = (rd.f64(E * NB).reshape(E, NB) for _ in range(4))
+    iters, res = rd.i64(), rd.f64(1)[0]
+    assert rel_err(M, oracle.mass(m, order)) < 1e-14
+    diag, off, orhs = oracle.implicit_assemble(m, order, U)
+    assert rel_err(rhs, orhs + M / dt * U) < 1e-12
+    Ad = np.zeros((E * NB, E * NB))
+    for e in range(E):
+        Ad[e * NB:(e + 1) * NB, e * NB:(e + 1) * NB] += diag[e] + np.diag(M[e] / dt)
+    for f, (c0, c1) in enumerate(m["face_cells"]):
+        if c1 >= 0:
+            Ad[c0 * NB:(c0 + 1) * NB, c1 * NB:(c1 + 1) * NB] += off[f, 0]
+            Ad[c1 * NB:(c1 + 1) * NB, c0 * NB:(c0 + 1) * NB] += off[f, 1]
+    assert iters > 0 and res < 1e-13
+    assert rel_err(x.ravel(), np.linalg.solve(Ad, rhs.ravel())) < 1e-9
+    assert rel_err(Ax.ravel(), Ad @ x.ravel()) < 1e-12
     assert rd.o == len(rd.b)
