@@ -102,7 +102,7 @@ struct fvdg_ctx {
     // implicit path (allocated on first use): assembled block rows (5 slots), right-hand side,
     // per-unknown mass, block-Jacobi factors, GMRES and Picard work vectors
     DevBuf<double> imp_A, imp_rhs, imp_b, imp_Md, imp_LU, imp_V, imp_w, imp_z, imp_part, imp_xold, imp_dx;
-    DevBuf<int32_t> imp_col, imp_piv, imp_dslot, imp_flag;
+    DevBuf<int32_t> imp_col, imp_piv, imp_dslot, imp_flag, imp_self;
     bool imp_have = false;           // imp_A holds a system assembled on the current mesh and scheme
 
     ~fvdg_ctx() {

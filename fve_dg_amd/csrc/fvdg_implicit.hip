@@ -45,9 +45,11 @@ struct Sys {
     const int32_t* dslot = nullptr;  // per block row: slot of the diagonal block, -1 none
     const double* Md = nullptr;      // per-unknown mass of the shift (nullable)
     double shift = 0.0;
-    double* LU = nullptr;
+    double* LU = nullptr;            // LU factors (NB > kMaxInvBlock) or explicit inverses
     int32_t* piv = nullptr;
     int32_t* flag = nullptr;
+    const int32_t* self = nullptr;   // per block row: its own index (one-slot product with the inverses)
+    bool use_inverse() const { return NB <= kMaxInvBlock; }
     DevBuf<double>* V = nullptr;  // (restart + 1) * n
     DevBuf<double>* w = nullptr;  // n
     DevBuf<double>* z = nullptr;  // n
@@ -57,8 +59,15 @@ struct Sys {
 
 int sys_factor(const Sys& S) {
     HIP_TRY(hipMemsetAsync(S.flag, 0, sizeof(int32_t), S.s));
-    hipLaunchKernelGGL(k_block_lu, dim3((unsigned)S.nrows), dim3(256), 0, S.s, S.A, S.dslot, S.slots, S.Md, S.shift,
-                       S.LU, S.piv, S.NB, S.flag);
+    if (S.use_inverse()) {
+        const size_t lds = sizeof(double) * ((size_t)S.NB * S.NB + S.NB);
+        HIP_TRY(hipFuncSetAttribute((const void*)k_block_inverse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_block_inverse, dim3((unsigned)S.nrows), dim3(256), lds, S.s, S.A, S.dslot, S.slots, S.Md,
+                           S.shift, S.LU, S.NB, S.flag);
+    } else {
+        hipLaunchKernelGGL(k_block_lu, dim3((unsigned)S.nrows), dim3(256), 0, S.s, S.A, S.dslot, S.slots, S.Md,
+                           S.shift, S.LU, S.piv, S.NB, S.flag);
+    }
     HIP_TRY(hipGetLastError());
     int32_t f = 0;
     HIP_TRY(hipMemcpyAsync(&f, S.flag, sizeof(int32_t), hipMemcpyDeviceToHost, S.s));
@@ -73,8 +82,12 @@ void sys_matvec(const Sys& S, const double* x, double* y) {
 }
 
 void sys_precond(const Sys& S, const double* r, double* z) {
-    hipLaunchKernelGGL(k_block_lu_solve, dim3((unsigned)S.nrows), dim3(256), sizeof(double) * S.NB, S.s, S.LU, S.piv,
-                       r, z, S.NB);
+    if (S.use_inverse())
+        hipLaunchKernelGGL(k_bsr_matvec, dim3(nblk(S.n(), 256)), dim3(256), 0, S.s, S.LU, S.self, 1, nullptr, 0.0, r,
+                           z, S.nrows, S.NB);
+    else
+        hipLaunchKernelGGL(k_block_lu_solve, dim3((unsigned)S.nrows), dim3(256), sizeof(double) * S.NB, S.s, S.LU,
+                           S.piv, r, z, S.NB);
 }
 
 // h[i] = V_i . w for i < nv (V_i consecutive vectors of length n starting at V)
@@ -223,8 +236,11 @@ int prepare(fvdg_ctx* c) {
         HIP_TRY(c->imp_flag.alloc(1));
         HIP_TRY(c->imp_xold.alloc((size_t)E * NB));
         HIP_TRY(c->imp_dx.alloc((size_t)E * NB));
-        // diagonal slot 0 in every row; per-unknown mass |det| mhat_k
+        // diagonal slot 0 in every row, each row's own index; per-unknown mass |det| mhat_k
         HIP_TRY(c->imp_dslot.upload(std::vector<int32_t>(E, 0), c->stream));
+        std::vector<int32_t> self(E);
+        for (int64_t e = 0; e < E; ++e) self[e] = (int32_t)e;
+        HIP_TRY(c->imp_self.upload(self, c->stream));
         std::vector<double> Md((size_t)E * NB);
         for (int64_t e = 0; e < E; ++e)
             for (int r = 0; r < NB; ++r) Md[(size_t)e * NB + r] = c->cell_det_host[e] * c->tab.mass_hat[r / 5];
@@ -277,6 +293,7 @@ Sys context_sys(fvdg_ctx* c, double shift) {
     S.LU = c->imp_LU.p;
     S.piv = c->imp_piv.p;
     S.flag = c->imp_flag.p;
+    S.self = c->imp_self.p;
     S.V = &c->imp_V;
     S.w = &c->imp_w;
     S.z = &c->imp_z;
@@ -470,10 +487,13 @@ int fvdg_bsr_solve(int device, int64_t n_rows, int block, const int64_t* row_ptr
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{s};
     DevBuf<double> dA, dLU, dV, dw, dz, dpart, db, dx;
-    DevBuf<int32_t> dcol, ddslot, dpiv, dflag;
+    DevBuf<int32_t> dcol, ddslot, dpiv, dflag, dself;
     HIP_TRY(dA.upload(A, s));
     HIP_TRY(dcol.upload(col, s));
     HIP_TRY(ddslot.upload(dslot, s));
+    std::vector<int32_t> self(n_rows);
+    for (int64_t r = 0; r < n_rows; ++r) self[r] = (int32_t)r;
+    HIP_TRY(dself.upload(self, s));
     const int64_t n = n_rows * block;
     HIP_TRY(dLU.alloc((size_t)n_rows * nbb));
     HIP_TRY(dpiv.alloc((size_t)n));
@@ -491,6 +511,7 @@ int fvdg_bsr_solve(int device, int64_t n_rows, int block, const int64_t* row_ptr
     S.LU = dLU.p;
     S.piv = dpiv.p;
     S.flag = dflag.p;
+    S.self = dself.p;
     S.V = &dV;
     S.w = &dw;
     S.z = &dz;

@@ -422,6 +422,91 @@ __global__ __launch_bounds__(256) void k_block_lu(const double* __restrict__ A, 
     }
 }
 
+// Explicit inverse of one NB x NB block per workgroup (NB <= kMaxInvBlock), by in-place Gauss-Jordan
+// elimination with partial pivoting in LDS (row-major there), of the block row's diagonal block
+// plus shift diag(Md); written column-major to Inv so that applying it is a block product
+// (k_bsr_matvec with one slot).  Row interchanges are undone on the columns at the end.
+constexpr int kMaxInvBlock = 100;
+__global__ __launch_bounds__(256) void k_block_inverse(const double* __restrict__ A, const int32_t* __restrict__ diag_slot,
+                                                       int slots, const double* __restrict__ Md, double shift,
+                                                       double* __restrict__ Inv, int NB, int32_t* __restrict__ singular) {
+    extern __shared__ double sA[];  // NB * NB row-major, then NB column factors
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    __shared__ int spiv[kMaxInvBlock];
+    double* sf = sA + NB * NB;
+    const int64_t b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int ds = diag_slot[b];
+    const double* src = ds >= 0 ? A + (b * slots + ds) * (int64_t)NB * NB : nullptr;
+    for (int t = tid; t < NB * NB; t += 256) {  // src column-major: t = col * NB + row
+        const int c = t / NB, r = t % NB;
+        double x = src ? src[t] : (r == c ? 1.0 : 0.0);
+        if (Md && src && r == c) x += shift * Md[b * NB + r];
+        sA[r * NB + c] = x;
+    }
+    __syncthreads();
+    for (int k = 0; k < NB; ++k) {
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + tid; i < NB; i += 256) {
+            const double x = fabs(sA[i * NB + k]);
+            if (x > best) {
+                best = x;
+                bi = i;
+            }
+        }
+        sv[tid] = best;
+        si[tid] = bi;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (tid < h && (sv[tid + h] > sv[tid] || (sv[tid + h] == sv[tid] && si[tid + h] < si[tid]))) {
+                sv[tid] = sv[tid + h];
+                si[tid] = si[tid + h];
+            }
+            __syncthreads();
+        }
+        const int p = si[0];
+        if (tid == 0) {
+            spiv[k] = p;
+            if (!(sv[0] > 0.0)) singular[0] = 1;  // benign race: every writer stores 1
+        }
+        if (p != k)
+            for (int j = tid; j < NB; j += 256) {
+                const double t = sA[k * NB + j];
+                sA[k * NB + j] = sA[p * NB + j];
+                sA[p * NB + j] = t;
+            }
+        __syncthreads();
+        const double pinv = 1.0 / sA[k * NB + k];
+        for (int i = tid; i < NB; i += 256) sf[i] = (i == k) ? 0.0 : sA[i * NB + k];
+        __syncthreads();
+        if (tid == 0) sA[k * NB + k] = 1.0;
+        __syncthreads();
+        for (int j = tid; j < NB; j += 256) sA[k * NB + j] *= pinv;
+        for (int i = tid; i < NB; i += 256)
+            if (i != k) sA[i * NB + k] = 0.0;
+        __syncthreads();
+        for (int t = tid; t < NB * NB; t += 256) {
+            const int i = t / NB, j = t % NB;
+            if (i != k) sA[t] -= sf[i] * sA[k * NB + j];
+        }
+        __syncthreads();
+    }
+    for (int k = NB - 1; k >= 0; --k) {
+        const int p = spiv[k];
+        if (p != k)
+            for (int i = tid; i < NB; i += 256) {
+                const double t = sA[i * NB + k];
+                sA[i * NB + k] = sA[i * NB + p];
+                sA[i * NB + p] = t;
+            }
+        __syncthreads();
+    }
+    double* out = Inv + b * (int64_t)NB * NB;
+    for (int t = tid; t < NB * NB; t += 256) out[t] = sA[(t % NB) * NB + t / NB];
+}
+
 // z = P^{-1} r per block row (forward and back substitution with the pivots of k_block_lu)
 __global__ __launch_bounds__(256) void k_block_lu_solve(const double* __restrict__ LU, const int32_t* __restrict__ piv,
                                                         const double* __restrict__ r, double* __restrict__ z, int NB) {

@@ -138,7 +138,7 @@ int alloc_state(fvdg_ctx* c) {
     for (DevBuf<double>* b : {&c->imp_A, &c->imp_rhs, &c->imp_b, &c->imp_Md, &c->imp_LU, &c->imp_V, &c->imp_w,
                               &c->imp_z, &c->imp_part, &c->imp_xold, &c->imp_dx})
         b->release();
-    for (DevBuf<int32_t>* b : {&c->imp_col, &c->imp_piv, &c->imp_dslot, &c->imp_flag}) b->release();
+    for (DevBuf<int32_t>* b : {&c->imp_col, &c->imp_piv, &c->imp_dslot, &c->imp_flag, &c->imp_self}) b->release();
     c->imp_have = false;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->have_state = false;

@@ -4,8 +4,8 @@
 //     EigenSparseSolver<DoFs, DoFs> solver(sparse_mat, rhs);        // sparse_mat finalized
 //     LongVector<DoFs> x = solver.DGMRES(x0);                      // Sedov.cpp:179-180
 //
-// Every method runs restarted GMRES (Saad & Schultz 1986) with a block-Jacobi preconditioner (LU of
-// each diagonal block) on the device (fvdg_bsr_solve) from the initial guess x0, to the relative
+// Every method runs restarted GMRES (Saad & Schultz 1986) with a block-Jacobi preconditioner (the
+// inverse or LU factors of each diagonal block) on the device (fvdg_bsr_solve) from the initial guess x0, to the relative
 // residual |b - A x| / |b| < tol (set_tol, default 1e-12) within set_iterations iterations (default
 // twice the number of unknowns) and restart length set_restart (default 30).  The reference's DGMRES
 // is Eigen's deflated GMRES with a diagonal preconditioner; SparseLU and BiCGSTAB are answered with

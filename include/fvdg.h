@@ -185,7 +185,8 @@ int fvdg_implicit_blocks(fvdg_ctx* c, double* diag, double* off, int32_t* off_co
 /* y = (A + shift M) x, M the diagonal mass (host vectors, n_cells x 5N) */
 int fvdg_implicit_matvec(fvdg_ctx* c, double shift, const double* x, double* y);
 /* solve (A + shift M) x = b (b NULL: the assembled rhs) by restarted GMRES(restart) with a
- * block-Jacobi preconditioner (LU of each diagonal block + shift M), from the guess in x, to a
+ * block-Jacobi preconditioner (each diagonal block + shift M inverted explicitly for NB <= 100,
+ * LU-factored beyond), from the guess in x, to a
  * relative residual |b - (A + shift M) x| / |b| < tol or max_iters iterations; host vectors.
  * iters / rel_res (nullable): iterations done and the final true relative residual.
  * Returns FVDG_ERR_NONFINITE when a diagonal block is singular. */

@@ -34,3 +34,11 @@ def test_vortex_errors():
     e64, e32 = np.array(r["fp64"]["analytic_l2_rel"])[keep], np.array(r["fp32"]["analytic_l2_rel"])[keep]
     assert np.isfinite(e64).all() and e64[0] < 0.05
     assert np.abs(e32 - e64).max() < 1e-6
+
+
+def test_implicit_steps_track_explicit():
+    print("[case] validate implicit n=4 P1", flush=True)
+    r = validate.run_implicit(n=4, order=1, steps=2, jitter=0.1, dt_factor=4.0, tol=1e-10)
+    assert all(p["picard_iters"] >= 1 and p["linear_iters"] >= 1 for p in r["per_step"])
+    ei, ee = np.array(r["analytic_l2_rel_implicit"]), np.array(r["analytic_l2_rel_explicit"])
+    assert np.isfinite(ei).all() and ei[0] < 0.2 and r["l2_rel_implicit_vs_explicit"] < 0.05
