@@ -557,10 +557,12 @@ struct FaceFluxArgs {
     int64_t ntiles;
 };
 
-template <int ORDER>
+template <int ORDER, typename T = double>
 struct FaceFluxShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF;
-    static constexpr int GP = 2, LPF = (QF + GP - 1) / GP;  // lanes per face
+    // face points per lane: two in FP64 (the trace products reuse each staged coefficient twice),
+    // one in FP32 (fewer registers per lane pay off there; measured at P3)
+    static constexpr int GP = std::is_same<T, double>::value ? 2 : 1, LPF = (QF + GP - 1) / GP;  // lanes per face
     // P5's table (8 x 36 x 57 doubles, 131 KB) leaves room for only a small tile
     static constexpr int THREADS = N > 35 ? 64 : 192;
     static constexpr int FB = THREADS / LPF;                // faces per tile
@@ -568,11 +570,13 @@ struct FaceFluxShape {
 
 template <int ORDER, typename T>
 __global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs<T> a) {
-    using SH = FaceFluxShape<ORDER>;
+    using SH = FaceFluxShape<ORDER, T>;
     constexpr bool SPLIT = !std::is_same<T, double>::value;
     constexpr int N = SH::N, QF = SH::QF, NV = 5 * N, GP = SH::GP, LPF = SH::LPF, FB = SH::FB;
-    constexpr int NT = SH::THREADS, NS = N | 1;  // odd table row stride: spreads LDS banks
-    __shared__ T sPhi[8 * QF * NS];
+    // tables transposed to [table][mode][point] with an odd point stride: at one mode the lanes of
+    // a face read within one row, and the eight tables start on well-separated LDS banks
+    constexpr int NT = SH::THREADS, QS = QF | 1;
+    __shared__ T sPhi[8 * N * QS];
     __shared__ T sC[2 * FB * NV];
     __shared__ double sC0[SPLIT ? 2 * FB * 5 : 1], sRef[SPLIT ? FB * 5 : 1];
     __shared__ int sPerm[3 * QF];
@@ -580,7 +584,10 @@ __global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs<T> a) {
     __shared__ int sCell[2 * FB], sCb[2 * FB], sKind[FB];
 
     const int t = threadIdx.x;
-    for (int i = t; i < 8 * QF * N; i += NT) sPhi[(i / N) * NS + (i % N)] = a.PhiC[i];
+    for (int i = t; i < 8 * QF * N; i += NT) {  // PhiC[c][g][k] -> sPhi[c][k][g]
+        const int c = i / (QF * N), g = (i / N) % QF, kk = i % N;
+        sPhi[(c * N + kk) * QS + g] = a.PhiC[i];
+    }
     for (int i = t; i < 3 * QF; i += NT) sPerm[i] = a.perm[i];
     if (t < 24) sCanon[t] = a.canon[t];
     const int fl = t / LPF, pg = t - fl * LPF;
@@ -630,8 +637,8 @@ __global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs<T> a) {
             T UL[GP][5], UR[GP][5];
             int gl[GP], gr[GP];
             const int cL = sCb[2 * fl], cR = sCb[2 * fl + 1];
-            const T* PL = sPhi + (cL / 3) * QF * NS;
-            const T* PR = sPhi + (cR / 3) * QF * NS;
+            const T* PL = sPhi + (cL / 3) * N * QS;
+            const T* PR = sPhi + (cR / 3) * N * QS;
 #pragma unroll
             for (int i = 0; i < GP; ++i) {
                 const int g = min(pg * GP + i, QF - 1);
@@ -653,7 +660,7 @@ __global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs<T> a) {
                     }
 #pragma unroll
                     for (int i = 0; i < GP; ++i) {
-                        const T pl = PL[gl[i] * NS + k], pr = PR[gr[i] * NS + k];
+                        const T pl = PL[k * QS + gl[i]], pr = PR[k * QS + gr[i]];
 #pragma unroll
                         for (int v = 0; v < 5; ++v) {
                             UL[i][v] += pl * c[v];

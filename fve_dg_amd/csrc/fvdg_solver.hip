@@ -289,11 +289,11 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
                     const T* PhiC, T* Fh, double* ms_flux, double* ms_test) {
     FaceFluxArgs<T> f{c->F, c->gamma, c->flux, U, c->fcell.p, c->fcombo.p, c->fgeo.p, c->fbc.p, c->fdir.p,
                       c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0};
-    constexpr int FB = FaceFluxShape<P>::FB;
+    constexpr int FB = FaceFluxShape<P, T>::FB;
     f.ntiles = (c->F + FB - 1) / FB;
     const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * c->grid_mult);
     Span s1(ms_flux, c->stream);
-    hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P>::THREADS), 0, c->stream, f);
+    hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P, T>::THREADS), 0, c->stream, f);
     s1.stop();
     constexpr int TE = FaceTestShape<P>::TE;
     FaceTestArgs<T> t{};
