@@ -60,6 +60,11 @@ TRAFFIC = {
 }
 
 
+def test_kernel_name(precision, order):
+    """The FP64 test pass runs on the matrix cores up to P4, the FP32 one and P5 on the LDS kernel."""
+    return "k_face_test_mfma" if precision == "fp64" and order <= 4 else "k_face_test_lds"
+
+
 def algorithmic(Q, Qf, N, word=8):
     """Algorithmic work per element and stage (FMA = 2 flops).
     volume kernel      40QN + 145Q flops (interpolation 10QN, test 30QN, pointwise flux ~145Q);
@@ -106,7 +111,7 @@ def cpu_baseline(workload_n, order, flux, field, L, seconds_target=10.0):
                       f"{sum(times):.1f} s total"}
 
 
-def kernel_roofline(workload, ms, E, Q, Qf, N, precision):
+def kernel_roofline(workload, ms, E, Q, Qf, N, precision, order):
     vol_f, flux_f, test_b = algorithmic(Q, Qf, N, 8 if precision == "fp64" else 4)
     peak_mm = PEAK_FP64 if precision == "fp64" else PEAK_FP32
 
@@ -118,8 +123,9 @@ def kernel_roofline(workload, ms, E, Q, Qf, N, precision):
                                 "unit": "TFLOP/s", "scale": 1e12},
         "face flux (k_face_flux_lds)": {"ms": ms[2], "achieved": rate(flux_f, ms[2]), "peak": PEAK_FP64,
                                         "bound": "mfma", "unit": "TFLOP/s", "scale": 1e12},
-        "face test + update (k_face_test_lds)": {"ms": ms[3], "achieved": rate(test_b, ms[3]), "peak": PEAK_HBM,
-                                                 "bound": "hbm", "unit": "GB/s", "scale": 1e9},
+        f"face test + update ({test_kernel_name(precision, order)})": {
+            "ms": ms[3], "achieved": rate(test_b, ms[3]), "peak": PEAK_HBM, "bound": "hbm", "unit": "GB/s",
+            "scale": 1e9},
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     k = kernels[dom]
@@ -156,7 +162,7 @@ def run_single(args, w):
           file=sys.stderr, flush=True)
     dof = 5 * N * E
     res = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16x3"):
         s.set_scheme(order, flux=flux, precision=prec)
         s.upload(U0)
         if args.warmup > 0:
@@ -178,8 +184,12 @@ def run_single(args, w):
     ms_lim = s.time_stages(dt, 3 * args.steps)
     s.set_stage_limiter(0)
     s.close()
-    f64, f32 = res["fp64"], res["fp32"]
-    dev = float(np.sqrt(((f32["U"] - f64["U"]) ** 2).sum() / (f64["U"] ** 2).sum()))
+    f64, f32, b3 = res["fp64"], res["fp32"], res["bf16x3"]
+
+    def deviation(r):
+        return float(np.sqrt(((r["U"] - f64["U"]) ** 2).sum() / (f64["U"] ** 2).sum()))
+
+    dev = deviation(f32)
     out = {
         "metric": METRIC,
         "value": round(f64["value"], 1),
@@ -196,12 +206,18 @@ def run_single(args, w):
                 f"on the mesh)",
         "config": {"workload": args.workload, "tets": E, "order": order, "modes": N, "flux": flux, "dofs": dof,
                    "stages_per_step": 3, "dt": dt, "cfl": 0.3, "wall_s": round(f64["wall"], 3)},
-        "roofline": kernel_roofline(args.workload, f64["ms"], E, Q, Qf, N, "fp64"),
+        "roofline": kernel_roofline(args.workload, f64["ms"], E, Q, Qf, N, "fp64", order),
         "mixed": {"precision": "fp32 contractions, fp64 fluxes/update/state",
                   "value": round(f32["value"], 1), "ms_per_step": round(3 * f32["ms"][0], 4),
                   "speedup_vs_fp64": round(f32["value"] / f64["value"], 4),
                   "l2_rel_deviation_vs_fp64": dev, "after_steps": args.warmup + args.steps,
-                  "roofline": kernel_roofline(args.workload, f32["ms"], E, Q, Qf, N, "fp32")},
+                  "roofline": kernel_roofline(args.workload, f32["ms"], E, Q, Qf, N, "fp32", order)},
+        "mixed_bf16x3": {"precision": "volume contractions as 6 bf16 MFMA products of 3-part splits (fp32 "
+                                      "accumulation), face contractions fp32, fp64 fluxes/update/state",
+                         "value": round(b3["value"], 1), "ms_per_step": round(3 * b3["ms"][0], 4),
+                         "speedup_vs_fp64": round(b3["value"] / f64["value"], 4),
+                         "l2_rel_deviation_vs_fp64": deviation(b3), "after_steps": args.warmup + args.steps,
+                         "volume_kernel_ms": round(b3["ms"][1], 4)},
         "limited": {"limiter": "PositiveLimiter apply_1 + apply_2 after every stage (fp64)",
                     "ms_per_step": round(3 * ms_lim[0], 4),
                     "overhead_vs_fp64": round(ms_lim[0] / f64["ms"][0] - 1, 4)},

@@ -79,6 +79,7 @@ struct fvdg_ctx {
     DevBuf<double> tPhiC;                 // canonical face tables (8 vertex orders)
     DevBuf<float> tVA32, tGA32, tPhiC32;  // FP32 copies for the mixed-precision kernels
     DevBuf<double> tGs, tIC;              // mixed: sum_q w_q grad phi_j (N*3), sum_g w_g PhiC (8*N)
+    DevBuf<uint16_t> tVB3, tGB3;          // BF16x3: volume operand tables split into 3 bf16 parts
     DevBuf<uint8_t> tCanon;               // per combo: canonical index * 3 + rotation
     DevBuf<int32_t> tPerm;                // face-rule point permutations of the 3 rotations
     int n_cu = 256;
@@ -89,6 +90,7 @@ struct fvdg_ctx {
     DevBuf<double> Fref;             // mixed precision: per-face reference flux F*5
     bool simple = false;             // FVDG_KERNELS=simple
     bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
+    bool lds_face_test = false;      // FVDG_FACE_TEST=lds: the LDS/VALU FP64 face test kernel
     DevBuf<double> red;              // reduction partials
     DevBuf<double> edt;              // per-element time step (ensembles); empty: the call's dt
     int64_t n_red = 0;

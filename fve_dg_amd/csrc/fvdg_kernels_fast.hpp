@@ -863,3 +863,137 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs<T> a) {
         }
     }
 }
+
+// ------------------------------------------------------------------ face test on the matrix cores
+// k_face_test_mfma<ORDER,FUSE>: the test pass of k_face_test_lds as 16x16x4 FP64 MFMA products.
+// One wavefront = 16 elements (the A rows); the K index is the canonical face point g' of
+// one canonical table c, the columns are test functions j:
+//   R[e][j][v] += sum_slots sum_g' Fh_slot[pinv_m(g')][v] * PhiC[c][g'][j]
+// where slot s of element e sees its face through table c(e,s) and rotation m(e,s), and pinv_m
+// is the inverse point permutation of that rotation (face point -> canonical point is perm_m).
+// Per slot the tile's elements use few tables (two on a conforming tet mesh: the face opposite
+// vertex s in either orientation), so for each table in use (wave-uniform ballot) the product
+// runs with the A rows of elements on another table set to zero.  The B operands (tables,
+// zero padded to whole 16x4 tiles) are laid out in LDS per lane: sB[c][ks][jt][lane].  The A
+// operands come straight from the face-flux array (each lane's 5 variables of a point are
+// contiguous), with the side's sign.  All products accumulate in the MFMA registers; the
+// epilogue adds Rv and applies the fused SSP-RK stage update exactly as k_face_test_lds.
+template <int ORDER>
+struct FaceTestMfmaShape {
+    static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
+    static constexpr int KS = (QF + 3) / 4, JT = (N + 15) / 16;
+    static constexpr int WAVES = 4, THREADS = 64 * WAVES, TE = 16;
+    static constexpr int LDS_B = 8 * KS * JT * 64;  // doubles
+    static constexpr bool SUPPORTED = LDS_B * 8 <= 80 * 1024;
+};
+
+template <int ORDER, bool FUSE>
+__global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> a) {
+    using SH = FaceTestMfmaShape<ORDER>;
+    using M = Mfma<double>;
+    typedef M::acc_t acc_t;
+    constexpr int N = SH::N, QF = SH::QF, NV = SH::NV, KS = SH::KS, JT = SH::JT;
+    __shared__ double sB[SH::LDS_B];
+    __shared__ int sPinv[3 * QF];
+    __shared__ int sCanon[24];
+    const int t = threadIdx.x, l = t & 63, wave = t >> 6, row = l & 15, kq = l >> 4;
+    for (int i = t; i < SH::LDS_B; i += SH::THREADS) {
+        const int ln = i & 63, rest = i >> 6, jt = rest % JT, ks = (rest / JT) % KS, c = rest / (JT * KS);
+        const int g = 4 * ks + (ln >> 4), j = 16 * jt + (ln & 15);
+        sB[i] = (g < QF && j < N) ? a.PhiC[((size_t)c * QF + g) * N + j] : 0.0;
+    }
+    for (int i = t; i < 3 * QF; i += SH::THREADS) {
+        const int m = i / QF, g = i - m * QF;
+        sPinv[m * QF + a.perm[i]] = g;
+    }
+    if (t < 24) sCanon[t] = a.canon[t];
+    __syncthreads();
+    const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
+
+    for (int64_t tile = (int64_t)blockIdx.x * SH::WAVES + wave; tile < a.ntiles;
+         tile += (int64_t)gridDim.x * SH::WAVES) {
+        const int64_t e0 = tile * SH::TE, er = e0 + row;
+        acc_t acc[5][JT];
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) acc[v][jt] = (acc_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s) {
+            // this lane's A row: element er, slot s
+            int cl = -1, m = 0;
+            int64_t f = 0;
+            double sign = 1.0;
+            if (er < a.E) {
+                const int32_t slot = a.cfaces[er * 4 + s];
+                const int64_t ff = slot >> 1;
+                const int side = slot & 1;
+                const bool bnd = a.fcell[2 * ff + 1] < 0;
+                if ((bnd && do_bnd) || (!bnd && do_int)) {
+                    const int cb = sCanon[a.fcombo[2 * ff + side]];
+                    cl = cb / 3;
+                    m = cb - 3 * cl;
+                    f = ff;
+                    sign = side ? -1.0 : 1.0;
+                }
+            }
+            // tables in use by the tile's elements at this slot (wave-uniform)
+            unsigned used = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) used |= (__ballot(cl == c) != 0 ? 1u : 0u) << c;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int gc = 4 * ks + kq;
+                double x[5];
+                if (cl >= 0 && gc < QF) {
+                    const double* p = a.Fh + (f * QF + sPinv[m * QF + gc]) * 5;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) x[v] = sign * p[v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) x[v] = 0.0;
+                }
+                for (unsigned u = used; u != 0; u &= u - 1) {
+                    const int c = __builtin_ctz(u);
+                    const bool mine = cl == c;
+                    double b[JT];
+#pragma unroll
+                    for (int jt = 0; jt < JT; ++jt) b[jt] = sB[((c * KS + ks) * JT + jt) * 64 + l];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        const double av = mine ? x[v] : 0.0;
+#pragma unroll
+                        for (int jt = 0; jt < JT; ++jt) acc[v][jt] = M::mma(av, b[jt], acc[v][jt]);
+                    }
+                }
+            }
+        }
+        // epilogue: accumulator element r of this lane is element e0 + row(r, kq), test function
+        // 16 jt + (l & 15)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t e = e0 + M::row(r, kq);
+            if (e >= a.E) continue;
+            const double det = FUSE ? a.cgeo[e * 10 + 9] : 0.0;
+            const double dte = FUSE ? (a.edt ? a.edt[e] : a.dt) : 0.0;
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) {
+                const int j = 16 * jt + (l & 15);
+                if (j >= N) continue;
+                const int64_t i0 = e * NV + 5 * j;
+                if (FUSE) {
+                    const double Mj = det * a.mhat[j];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        const double Rt = a.Rv[i0 + v] + acc[v][jt][r];
+                        const double sv = a.U[i0 + v] + dte * (-Rt / Mj);
+                        a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) a.R[i0 + v] = a.Rv ? a.Rv[i0 + v] + acc[v][jt][r] : acc[v][jt][r];
+                }
+            }
+        }
+    }
+}

@@ -28,6 +28,7 @@
 #include "fvdg_kernels_fast.hpp"
 #include "fvdg_limiter.hpp"
 #include "fvdg_diagnostics.hpp"
+#include "fvdg_bf16x3.hpp"
 
 using namespace fvdg;
 
@@ -42,6 +43,68 @@ int check_ready(fvdg_ctx* c, bool need_state) {
 }
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// bf16 bits of a float, rounded to nearest even
+uint16_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+double bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+// x = x0 + x1 + x2 in bf16 parts, written to parts[p * stride + i]
+void split_bf16x3(double x, std::vector<uint16_t>& parts, size_t stride, size_t i) {
+    for (int p = 0; p < 3; ++p) {
+        const uint16_t b = bf16_bits((float)x);
+        parts[p * stride + i] = b;
+        x -= bf16_value(b);
+    }
+}
+
+// BF16x3 operand tables of the volume kernel, in the per-lane order of fvdg_bf3::k_vol_bf16x3
+template <int P>
+int upload_bf16x3_tables_t(fvdg_ctx* c) {
+    using S = fvdg_bf3::Bf3Shape<P>;
+    const SchemeTables& T = c->tab;
+    const int N = T.N, Q = T.Q;
+    std::vector<uint16_t> VB(3 * (size_t)S::VB, 0), GB(3 * (size_t)S::GB, 0);
+    for (int t = 0; t < S::QT; ++t)
+        for (int ks = 0; ks < S::KS; ++ks)
+            for (int l = 0; l < 64; ++l)
+                for (int i = 0; i < 8; ++i) {
+                    const int q = 16 * t + l % 16, k = 32 * ks + 8 * (l / 16) + i;
+                    const double x = (q < Q && k < N) ? T.V[q * N + k] : 0.0;
+                    split_bf16x3(x, VB, S::VB, (((size_t)t * S::KS + ks) * 64 + l) * 8 + i);
+                }
+    for (int t = 0; t < S::QT; ++t)
+        for (int s = 0; s < 2; ++s)
+            for (int jt = 0; jt < S::JT; ++jt)
+                for (int l = 0; l < 64; ++l)
+                    for (int i = 0; i < 8; ++i) {
+                        const int pr = 8 * s + i, r = pr / 3, d = pr % 3;
+                        const int j = 16 * jt + l % 16, q = 16 * t + 4 * (l / 16) + r;
+                        const double x = (pr < 12 && j < N && q < Q) ? T.G[(q * 3 + d) * N + j] : 0.0;
+                        split_bf16x3(x, GB, S::GB, ((((size_t)t * 2 + s) * S::JT + jt) * 64 + l) * 8 + i);
+                    }
+    HIP_TRY(c->tVB3.upload(VB, c->stream));
+    HIP_TRY(c->tGB3.upload(GB, c->stream));
+    return FVDG_OK;
+}
+int upload_bf16x3_tables(fvdg_ctx* c) {
+    switch (c->order) {
+        case 0: return upload_bf16x3_tables_t<0>(c);
+        case 1: return upload_bf16x3_tables_t<1>(c);
+        case 2: return upload_bf16x3_tables_t<2>(c);
+        case 3: return upload_bf16x3_tables_t<3>(c);
+        case 4: return upload_bf16x3_tables_t<4>(c);
+        default: return upload_bf16x3_tables_t<5>(c);
+    }
+}
 
 int upload_tables(fvdg_ctx* c) {
     const SchemeTables& T = c->tab;
@@ -92,7 +155,7 @@ int upload_tables(fvdg_ctx* c) {
     HIP_TRY(c->tIC.upload(IC, c->stream));
     HIP_TRY(c->tCanon.upload(canon, c->stream));
     HIP_TRY(c->tPerm.upload(perm, c->stream));
-    return FVDG_OK;
+    return upload_bf16x3_tables(c);
 }
 
 // Scratch buffers of the contraction precision (face fluxes, and for the mixed path the
@@ -254,7 +317,11 @@ void launch_vol_t(fvdg_ctx* c, const double* U, double* R, const T* VA, const T*
 }
 template <int P>
 void launch_vol_fast(fvdg_ctx* c, const double* U, double* ext) {
-    if (mixed(c))
+    if (c->precision == FVDG_PREC_BF16X3 && !c->simple) {
+        fvdg_bf3::Bf3Args b{c->E, c->gamma, U, c->R.p, c->cgeo.p, reinterpret_cast<const __bf16*>(c->tVB3.p),
+                            reinterpret_cast<const __bf16*>(c->tGB3.p), c->tWqa.p, c->tGs.p};
+        hipLaunchKernelGGL((fvdg_bf3::k_vol_bf16x3<P>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, b);
+    } else if (mixed(c))
         launch_vol_t<P, float>(c, U, c->R.p, c->tVA32.p, c->tGA32.p, nullptr);
     else
         launch_vol_t<P, double>(c, U, c->R.p, c->tVA.p, c->tGA.p, ext);
@@ -321,6 +388,22 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.PhiC = PhiC;
     t.canon = c->tCanon.p;
     t.perm = c->tPerm.p;
+    if constexpr (std::is_same<T, double>::value && FaceTestMfmaShape<P>::SUPPORTED) {
+        if (!c->lds_face_test) {
+            using MS = FaceTestMfmaShape<P>;
+            t.ntiles = (c->E + MS::TE - 1) / MS::TE;
+            const auto kern = up ? k_face_test_mfma<P, true> : k_face_test_mfma<P, false>;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, MS::THREADS, 0) != hipSuccess || per_cu < 1)
+                per_cu = 1;
+            const unsigned g2 = (unsigned)std::min<int64_t>((t.ntiles + MS::WAVES - 1) / MS::WAVES,
+                                                            (int64_t)c->n_cu * per_cu);
+            Span s2(ms_test, c->stream);
+            hipLaunchKernelGGL(kern, dim3(g2), dim3(MS::THREADS), 0, c->stream, t);
+            s2.stop();
+            return;
+        }
+    }
     t.ntiles = (c->E + TE - 1) / TE;
     const unsigned g2 = (unsigned)std::min<int64_t>(t.ntiles, (int64_t)c->n_cu * c->grid_mult);
     Span s2(ms_test, c->stream);
@@ -530,6 +613,8 @@ int fvdg_create(fvdg_ctx** out, int device) {
     c->simple = kern && std::strcmp(kern, "simple") == 0;
     const char* faces = std::getenv("FVDG_FACES");
     c->owner_faces = faces && std::strcmp(faces, "owner") == 0;
+    const char* ftest = std::getenv("FVDG_FACE_TEST");
+    c->lds_face_test = ftest && std::strcmp(ftest, "lds") == 0;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();
     return FVDG_OK;
@@ -547,8 +632,8 @@ int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precisio
     if (order < 0 || order > kMaxOrder) return fail(FVDG_ERR_UNSUPPORTED, "order must be 0..5");
     if (flux != FVDG_FLUX_LF && flux != FVDG_FLUX_ROE) return fail(FVDG_ERR_ARG, "unknown flux");
     if (!(gamma > 1.0)) return fail(FVDG_ERR_ARG, "gamma must exceed 1");
-    if (precision != FVDG_PREC_FP64 && precision != FVDG_PREC_FP32)
-        return fail(FVDG_ERR_UNSUPPORTED, "precision must be FVDG_PREC_FP64 or FVDG_PREC_FP32");
+    if (precision != FVDG_PREC_FP64 && precision != FVDG_PREC_FP32 && precision != FVDG_PREC_BF16X3)
+        return fail(FVDG_ERR_UNSUPPORTED, "precision must be FVDG_PREC_FP64, FVDG_PREC_FP32 or FVDG_PREC_BF16X3");
     HIP_TRY(hipSetDevice(c->device));
     const bool reorder = c->order != order;
     c->flux = flux;

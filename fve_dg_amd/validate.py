@@ -124,12 +124,14 @@ def run(kind, n, order, steps, every, jitter, flux="roe", cfl=0.3, precisions=("
             rec["analytic_l2_rel"] = mesh.analytic_error(order, field, params, steps * dt, U).tolist()
         out[prec] = rec
     s.close()
-    if "fp64" in states and "fp32" in states:
-        a, b = states["fp32"], states["fp64"]
-        out["l2_rel_deviation_fp32_vs_fp64"] = float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
-        s64, s32 = np.array(out["fp64"]["series"]), np.array(out["fp32"]["series"])
-        out["series_max_rel_deviation"] = {
-            name: float(np.max(np.abs(s32[:, i] - s64[:, i]) / np.maximum(np.abs(s64[:, i]), 1e-300)))
+    for prec in states:
+        if prec == "fp64" or "fp64" not in states:
+            continue
+        a, b = states[prec], states["fp64"]
+        out[f"l2_rel_deviation_{prec}_vs_fp64"] = float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+        s64, sm = np.array(out["fp64"]["series"]), np.array(out[prec]["series"])
+        out[f"series_max_rel_deviation_{prec}"] = {
+            name: float(np.max(np.abs(sm[:, i] - s64[:, i]) / np.maximum(np.abs(s64[:, i]), 1e-300)))
             for i, name in ((1, "kinetic_energy"), (2, "enstrophy"), (3, "mass"), (4, "total_energy"))}
     return out
 
@@ -144,6 +146,7 @@ def main(argv=None):
     ap.add_argument("--jitter", type=float)
     ap.add_argument("--flux", default="roe", choices=["lf", "roe"])
     ap.add_argument("--dt-factor", type=float, default=10.0, help="implicit: dt / explicit stable dt")
+    ap.add_argument("--precisions", default="fp64,fp32", help="tgv/vortex: comma-separated, fp64 first")
     args = ap.parse_args(argv)
     cfg = dict(DEFAULTS[args.kind])
     for k in cfg:
@@ -153,7 +156,7 @@ def main(argv=None):
         cfg.pop("every")
         print(json.dumps(run_implicit(dt_factor=args.dt_factor, **cfg)))
     else:
-        print(json.dumps(run(args.kind, flux=args.flux, **cfg)))
+        print(json.dumps(run(args.kind, flux=args.flux, precisions=tuple(args.precisions.split(",")), **cfg)))
 
 
 if __name__ == "__main__":

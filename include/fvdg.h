@@ -131,10 +131,12 @@ int fvdg_analytic_error(const fvdg_mesh* m, int order, int field, const double* 
 /* ---------------------------- device solver ---------------------------- */
 int fvdg_create(fvdg_ctx** out, int device);
 void fvdg_destroy(fvdg_ctx* c);
-/* order 0..5; flux FVDG_FLUX_*; gamma > 1; precision FVDG_PREC_FP64 (the drop-in) or FVDG_PREC_FP32
- * (volume and face contractions in FP32 with FP32 accumulation; pointwise and Riemann fluxes,
- * stage update, state and conservation sums stay FP64).  Changing only the precision keeps
- * an uploaded state. */
+/* order 0..5; flux FVDG_FLUX_*; gamma > 1; precision FVDG_PREC_FP64 (the drop-in),
+ * FVDG_PREC_FP32 (volume and face contractions in FP32 with FP32 accumulation) or
+ * FVDG_PREC_BF16X3 (volume contractions on the bf16 matrix cores: each FP32 operand split into
+ * three bf16 parts, six part products accumulated in FP32; face contractions as FP32).  In both
+ * mixed forms the pointwise and Riemann fluxes, stage update, state and conservation sums stay
+ * FP64.  Changing only the precision keeps an uploaded state. */
 int fvdg_set_scheme(fvdg_ctx* c, int order, int flux, double gamma, int precision);
 /* Copies the mesh geometry to the device; the mesh object may be destroyed afterwards. */
 int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m);

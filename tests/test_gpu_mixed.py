@@ -1,10 +1,12 @@
-"""GPU mixed precision (FVDG_PREC_FP32): volume and face contractions in FP32 with FP32
-accumulation, pointwise/Riemann fluxes, stage update and state in FP64.
+"""GPU mixed precision: FVDG_PREC_FP32 (volume and face contractions in FP32 with FP32
+accumulation) and FVDG_PREC_BF16X3 (the volume contractions on the bf16 matrix cores with
+three-part bf16 operands and FP32 accumulation, the face contractions as FP32); pointwise/Riemann
+fluxes, stage update and state in FP64 in both.
 
-Bars (written per test): the FP32-contraction residual is within 2e-5 (norm-wise relative)
-of the FP64 oracle residual and differs from it (the FP32 kernels really ran); a trajectory
-stays within 1e-6 relative L2 of the FP64 trajectory; its error against the analytic vortex
-equals the FP64 path's to 1e-3 relative; the cell-mean integrals stay conserved.
+Bars (written per test): the mixed residual is within 2e-5 (norm-wise relative) of the FP64
+oracle residual and differs from it (the mixed kernels really ran); a trajectory stays within
+1e-6 relative L2 of the FP64 trajectory; its error against the analytic vortex equals the FP64
+path's to 1e-3 relative; the cell-mean integrals stay conserved.
 """
 import numpy as np
 import pytest
@@ -19,14 +21,18 @@ def l2_rel(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
 
 
+MIXED = ["fp32", "bf16x3"]
+
+
+@pytest.mark.parametrize("prec", MIXED)
 @pytest.mark.parametrize("order", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("flux", ["lf", "roe"])
-def test_fp32_residual_close_to_oracle(solver, oracle, order, flux):
-    print(f"[case] fp32 residual order={order} flux={flux}", flush=True)
+def test_fp32_residual_close_to_oracle(solver, oracle, order, flux, prec):
+    print(f"[case] {prec} residual order={order} flux={flux}", flush=True)
     mesh = F.Mesh.box(3, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=21)
     U = mesh.project(order, "vortex", VORTEX)
     U *= 1 + 1e-2 * np.random.default_rng(order).standard_normal(U.shape)
-    solver.set_scheme(order, flux=flux, precision="fp32").set_mesh(mesh)
+    solver.set_scheme(order, flux=flux, precision=prec).set_mesh(mesh)
     solver.upload(U)
     Rm = solver.residual()
     Ro = oracle.residual(mesh.export(), order, U, flux=flux)
@@ -35,13 +41,14 @@ def test_fp32_residual_close_to_oracle(solver, oracle, order, flux):
     assert order == 0 or err > 1e-12  # FP32 contractions were used
 
 
-def test_fp32_parts_and_boundaries(solver, oracle):
+@pytest.mark.parametrize("prec", MIXED)
+def test_fp32_parts_and_boundaries(solver, oracle, prec):
     mesh = F.Mesh.box((4, 4, 2), ub=(10, 10, 2.5), split=5)
     mesh.tag_pseudo3dz()
     bc = mesh.boundary_types()
     params = [1.4, 5.0, 4.0, 4.5, 1.0, 0.5, 0.0, 0.0]
     U = mesh.project(3, "vortex", params)
-    solver.set_scheme(3, flux="roe", precision="fp32").set_mesh(mesh)
+    solver.set_scheme(3, flux="roe", precision=prec).set_mesh(mesh)
     solver.upload(U)
     _, xyz = solver.dirichlet_points()
     UD = np.array([[oracle.vortex_state(params, 0.0, x) for x in row] for row in xyz])
@@ -66,31 +73,52 @@ def test_precision_switch_keeps_the_state(solver):
     assert 0 < rel_err(r32, r64) < 2e-5
 
 
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_bf16x3_volume_is_its_own_kernel(solver, order):
+    """The BF16x3 volume term differs from the FP32 one in the last bits (another contraction
+    ran) and both stay within 2e-5 of the FP64 volume term, relative to its norm."""
+    print(f"[case] bf16x3 volume order={order}", flush=True)
+    mesh = F.Mesh.box(4, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=5)
+    U = mesh.project(order, "vortex", VORTEX)
+    U *= 1 + 1e-2 * np.random.default_rng(order).standard_normal(U.shape)
+    R = {}
+    for p in ("fp64", "fp32", "bf16x3"):
+        solver.set_scheme(order, flux="roe", precision=p).set_mesh(mesh)
+        solver.upload(U)
+        R[p] = solver.residual(parts=F.PART_CELLS)
+    assert not np.array_equal(R["bf16x3"], R["fp32"])
+    for p in ("fp32", "bf16x3"):
+        err = rel_err(R[p], R["fp64"])
+        print(f"{p} volume term vs fp64: {err:.3e}", flush=True)
+        assert 0 < err < 2e-5, p
+
+
+@pytest.mark.parametrize("prec", MIXED)
 @pytest.mark.parametrize("order,n,steps", [(3, 16, 40), (1, 16, 40)])
-def test_fp32_trajectory_tracks_fp64(solver, order, n, steps):
-    """`steps` SSP-RK3 steps of the jittered periodic vortex: FP32-contraction state within
+def test_fp32_trajectory_tracks_fp64(solver, order, n, steps, prec):
+    """`steps` SSP-RK3 steps of the jittered periodic vortex: mixed-precision state within
     1e-6 relative L2 of the FP64 state; same analytic error; conserved integrals."""
-    print(f"[case] fp32 trajectory order={order} n={n} steps={steps}", flush=True)
+    print(f"[case] {prec} trajectory order={order} n={n} steps={steps}", flush=True)
     mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=60)
     U0 = mesh.project(order, "vortex", VORTEX)
     out, cons = {}, {}
-    for prec in ("fp64", "fp32"):
-        solver.set_scheme(order, flux="roe", precision=prec).set_mesh(mesh)
+    for p in ("fp64", prec):
+        solver.set_scheme(order, flux="roe", precision=p).set_mesh(mesh)
         solver.upload(U0)
         dt = solver.stable_dt(0.3)
         c0 = solver.conservation_sums()
         solver.ssprk3(dt, steps)
-        cons[prec] = (c0, solver.conservation_sums())
-        out[prec] = (solver.download(), dt)
-    (U64, dt), (U32, _) = out["fp64"], out["fp32"]
+        cons[p] = (c0, solver.conservation_sums())
+        out[p] = (solver.download(), dt)
+    (U64, dt), (U32, _) = out["fp64"], out[prec]
     dev = l2_rel(U32, U64)
-    print(f"fp32 vs fp64 relative L2 after {steps} steps: {dev:.3e}", flush=True)
+    print(f"{prec} vs fp64 relative L2 after {steps} steps: {dev:.3e}", flush=True)
     assert dev < 1e-6
     t = steps * dt
     e64 = mesh.analytic_error(order, "vortex", VORTEX, t, U64)
     e32 = mesh.analytic_error(order, "vortex", VORTEX, t, U32)
     assert np.all(np.abs(e32 - e64) <= 1e-3 * np.abs(e64) + 1e-14)
-    for prec, (c0, c1) in cons.items():
+    for p, (c0, c1) in cons.items():
         # relative to the largest conserved integral (the rho w integral of this flow is zero)
         drift = np.abs(np.asarray(c1) - np.asarray(c0)) / np.abs(c0).max()
-        assert drift.max() < (1e-13 if prec == "fp64" else 1e-11), (prec, drift)
+        assert drift.max() < (1e-13 if p == "fp64" else 1e-11), (p, drift)

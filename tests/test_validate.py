@@ -1,6 +1,7 @@
 """The long-run validation driver (fve_dg_amd.validate) on small meshes: series recorded at the
-chunk ends, FP32 contractions tracking FP64 (kinetic energy, enstrophy, conserved integrals,
-final state), mass and total energy conserved, the vortex's error against the exact solution."""
+chunk ends, the FP32 and BF16x3 contractions tracking FP64 (kinetic energy, enstrophy, conserved
+integrals, final state), mass and total energy conserved, the vortex's error against the exact
+solution."""
 import numpy as np
 import pytest
 
@@ -11,8 +12,8 @@ pytestmark = pytest.mark.gpu
 
 def test_tgv_series():
     print("[case] validate tgv n=4 P2", flush=True)
-    r = validate.run("tgv", n=4, order=2, steps=20, every=5, jitter=0.0)
-    for prec in ("fp64", "fp32"):
+    r = validate.run("tgv", n=4, order=2, steps=20, every=5, jitter=0.0, precisions=("fp64", "fp32", "bf16x3"))
+    for prec in ("fp64", "fp32", "bf16x3"):
         S = np.array(r[prec]["series"])
         assert S.shape == (5, 7) and np.isfinite(S).all()
         assert abs(S[0, 1] - 0.125) < 5e-3  # E_k(0) of the projected field
@@ -22,8 +23,9 @@ def test_tgv_series():
         assert (np.abs(S[:, 3] - S[0, 3]) < tol * S[0, 3]).all(), prec  # mass
         assert (np.abs(S[:, 4] - S[0, 4]) < tol * S[0, 4]).all(), prec  # total energy
         assert S[-1, 1] <= S[0, 1]  # the scheme only dissipates kinetic energy
-    assert r["l2_rel_deviation_fp32_vs_fp64"] < 1e-7
-    assert max(r["series_max_rel_deviation"].values()) < 1e-7
+    for prec in ("fp32", "bf16x3"):
+        assert r[f"l2_rel_deviation_{prec}_vs_fp64"] < 1e-7, prec
+        assert max(r[f"series_max_rel_deviation_{prec}"].values()) < 1e-7, prec
 
 
 def test_vortex_errors():
