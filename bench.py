@@ -61,8 +61,10 @@ TRAFFIC = {
 
 
 def test_kernel_name(precision, order):
-    """The FP64 test pass runs on the matrix cores up to P4, the FP32 one and P5 on the LDS kernel."""
-    return "k_face_test_mfma" if precision == "fp64" and order <= 4 else "k_face_test_lds"
+    """The FP64 test pass runs on the matrix cores up to P4 (unless FVDG_FACE_TEST=lds), the FP32 one
+    and P5 on the LDS kernel."""
+    mfma = precision == "fp64" and order <= 4 and os.environ.get("FVDG_FACE_TEST") != "lds"
+    return "k_face_test_mfma" if mfma else "k_face_test_lds"
 
 
 def algorithmic(Q, Qf, N, word=8):

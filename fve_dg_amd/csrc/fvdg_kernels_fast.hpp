@@ -748,6 +748,7 @@ struct FaceTestArgs {
     const uint8_t* canon;
     const int32_t* perm;
     int64_t ntiles;
+    const uint8_t* cmeta;  // E*4: vertex combination of the slot's side | 0x80 on boundary faces
 };
 
 template <int ORDER>
@@ -918,6 +919,13 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
         for (int v = 0; v < 5; ++v)
 #pragma unroll
             for (int jt = 0; jt < JT; ++jt) acc[v][jt] = (acc_t){0.0, 0.0, 0.0, 0.0};
+        // the four slots of this lane's A-row element (one 16-byte and one 4-byte load)
+        int4 cf4 = make_int4(-1, -1, -1, -1);
+        uchar4 cm4 = make_uchar4(0, 0, 0, 0);
+        if (er < a.E) {
+            cf4 = *reinterpret_cast<const int4*>(a.cfaces + er * 4);
+            cm4 = *reinterpret_cast<const uchar4*>(a.cmeta + er * 4);
+        }
 #pragma unroll 1
         for (int s = 0; s < 4; ++s) {
             // this lane's A row: element er, slot s
@@ -925,16 +933,15 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
             int64_t f = 0;
             double sign = 1.0;
             if (er < a.E) {
-                const int32_t slot = a.cfaces[er * 4 + s];
-                const int64_t ff = slot >> 1;
-                const int side = slot & 1;
-                const bool bnd = a.fcell[2 * ff + 1] < 0;
+                const int32_t slot = s == 0 ? cf4.x : s == 1 ? cf4.y : s == 2 ? cf4.z : cf4.w;
+                const int meta = s == 0 ? cm4.x : s == 1 ? cm4.y : s == 2 ? cm4.z : cm4.w;
+                const bool bnd = (meta & 0x80) != 0;
                 if ((bnd && do_bnd) || (!bnd && do_int)) {
-                    const int cb = sCanon[a.fcombo[2 * ff + side]];
+                    const int cb = sCanon[meta & 0x1f];
                     cl = cb / 3;
                     m = cb - 3 * cl;
-                    f = ff;
-                    sign = side ? -1.0 : 1.0;
+                    f = slot >> 1;
+                    sign = (slot & 1) ? -1.0 : 1.0;
                 }
             }
             // tables in use by the tile's elements at this slot (wave-uniform)

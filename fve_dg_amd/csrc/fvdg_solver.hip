@@ -371,6 +371,7 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.R = Rout;
     t.cgeo = c->cgeo.p;
     t.cfaces = c->cfaces.p;
+    t.cmeta = c->cmeta.p;
     t.fcell = c->fcell.p;
     t.fcombo = c->fcombo.p;
     t.mhat = c->tMhat.p;
@@ -662,6 +663,7 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
     if (E > (int64_t(1) << 30) || F > (int64_t(1) << 30)) return fail(FVDG_ERR_UNSUPPORTED, "mesh too large for 32-bit indices");
     std::vector<double> cgeo(E * 10);
     std::vector<int32_t> cfaces(E * 4);
+    std::vector<uint8_t> cmeta(E * 4);
     std::vector<double> det(E);
     double h_min = INFINITY;
     for (int64_t e = 0; e < E; ++e) {
@@ -709,6 +711,7 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
                 side = has_opp ? 1 : 0;
             }
             cfaces[e * 4 + l] = (int32_t)(2 * f + side);
+            cmeta[e * 4 + l] = (uint8_t)(fcombo[2 * f + side] | (Fc.is_boundary() ? 0x80 : 0));
         }
     }
     c->E = E;
@@ -721,6 +724,7 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
     c->h_min = h_min;
     HIP_TRY(c->cgeo.upload(cgeo, c->stream));
     HIP_TRY(c->cfaces.upload(cfaces, c->stream));
+    HIP_TRY(c->cmeta.upload(cmeta, c->stream));
     HIP_TRY(c->fcell.upload(fcell, c->stream));
     HIP_TRY(c->fcombo.upload(fcombo, c->stream));
     HIP_TRY(c->fgeo.upload(fgeo, c->stream));
