@@ -563,13 +563,14 @@ struct FaceFluxShape {
     // face points per lane: two in FP64 (the trace products reuse each staged coefficient twice),
     // one in FP32 (fewer registers per lane pay off there; measured at P3)
     static constexpr int GP = std::is_same<T, double>::value ? 2 : 1, LPF = (QF + GP - 1) / GP;  // lanes per face
-    // P5's table (8 x 36 x 57 doubles, 131 KB) leaves room for only a small tile
-    static constexpr int THREADS = N > 35 ? 64 : 192;
+    // P5's table (8 x 36 x 57 doubles, 131 KB) leaves room for only a small tile; elsewhere 4
+    // wavefronts per block share one copy of the tables (P3: 2 blocks = 8 waves per CU)
+    static constexpr int THREADS = N > 35 ? 64 : 256;
     static constexpr int FB = THREADS / LPF;                // faces per tile
 };
 
 template <int ORDER, typename T>
-__global__ __launch_bounds__(192) void k_face_flux_lds(FaceFluxArgs<T> a) {
+__global__ __launch_bounds__(256) void k_face_flux_lds(FaceFluxArgs<T> a) {
     using SH = FaceFluxShape<ORDER, T>;
     constexpr bool SPLIT = !std::is_same<T, double>::value;
     constexpr int N = SH::N, QF = SH::QF, NV = 5 * N, GP = SH::GP, LPF = SH::LPF, FB = SH::FB;
