@@ -83,6 +83,7 @@ struct fvdg_ctx {
     DevBuf<uint16_t> tVB3, tGB3;          // BF16x3: volume operand tables split into 3 bf16 parts
     DevBuf<uint8_t> tCanon;               // per combo: canonical index * 3 + rotation
     DevBuf<int32_t> tPerm;                // face-rule point permutations of the 3 rotations
+    DevBuf<double> tPhiB;                 // canonical tables as MFMA B operands (face test)
     int n_cu = 256;
     // device state: U (state), Ua/Ub (stage ping-pong), R (residual)
     DevBuf<double> U, Ua, Ub, R;

@@ -750,6 +750,7 @@ struct FaceTestArgs {
     const int32_t* perm;
     int64_t ntiles;
     const uint8_t* cmeta;  // E*4: vertex combination of the slot's side | 0x80 on boundary faces
+    const double* PhiB;    // k_face_test_mfma: canonical tables as per-lane MFMA B operands
 };
 
 template <int ORDER>
@@ -875,19 +876,31 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs<T> a) {
 // is the inverse point permutation of that rotation (face point -> canonical point is perm_m).
 // Per slot the tile's elements use few tables (two on a conforming tet mesh: the face opposite
 // vertex s in either orientation), so for each table in use (wave-uniform ballot) the product
-// runs with the A rows of elements on another table set to zero.  The B operands (tables,
-// zero padded to whole 16x4 tiles) are laid out in LDS per lane: sB[c][ks][jt][lane].  The A
-// operands come straight from the face-flux array (each lane's 5 variables of a point are
-// contiguous), with the side's sign.  All products accumulate in the MFMA registers; the
-// epilogue adds Rv and applies the fused SSP-RK stage update exactly as k_face_test_lds.
+// runs with the A rows of elements on another table set to zero.  The B operands (tables, zero
+// padded to whole 16x4 tiles) are stored per lane, PhiB[c][ks][jt][lane]: one coalesced 512-byte
+// load per product, served by the caches.  The A operands come straight from the face-flux array
+// (each lane's 5 variables of a point are contiguous), with the side's sign.  The products
+// accumulate in registers; the wavefront then writes its 16 x 5N block to LDS in element-major
+// order and the epilogue (R_vol added, fused SSP-RK stage update exactly as k_face_test_lds)
+// reads and writes the state arrays in contiguous runs.
 template <int ORDER>
 struct FaceTestMfmaShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
     static constexpr int KS = (QF + 3) / 4, JT = (N + 15) / 16;
-    static constexpr int WAVES = 4, THREADS = 64 * WAVES, TE = 16;
-    static constexpr int LDS_B = 8 * KS * JT * 64;  // doubles
-    static constexpr bool SUPPORTED = LDS_B * 8 <= 80 * 1024;
+    // the epilogue block (16 x 5N doubles per wavefront) sets the LDS per block: from P4 on, one
+    // wavefront per block lets more of them share a compute unit
+    static constexpr int WAVES = N > 20 ? 1 : 4, THREADS = 64 * WAVES, TE = 16;
+    static constexpr int OUT = TE * NV;  // doubles of LDS per wavefront
+    static constexpr bool SUPPORTED = WAVES * OUT * 8 <= 96 * 1024;
 };
+
+// LDS ordering between the lanes of one wavefront (the waves of a block run different tile
+// counts, so a block barrier cannot be used inside the tile loop)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int ORDER, bool FUSE>
 __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> a) {
@@ -895,15 +908,10 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
     using M = Mfma<double>;
     typedef M::acc_t acc_t;
     constexpr int N = SH::N, QF = SH::QF, NV = SH::NV, KS = SH::KS, JT = SH::JT;
-    __shared__ double sB[SH::LDS_B];
+    __shared__ double sOut[SH::WAVES * SH::OUT];
     __shared__ int sPinv[3 * QF];
     __shared__ int sCanon[24];
     const int t = threadIdx.x, l = t & 63, wave = t >> 6, row = l & 15, kq = l >> 4;
-    for (int i = t; i < SH::LDS_B; i += SH::THREADS) {
-        const int ln = i & 63, rest = i >> 6, jt = rest % JT, ks = (rest / JT) % KS, c = rest / (JT * KS);
-        const int g = 4 * ks + (ln >> 4), j = 16 * jt + (ln & 15);
-        sB[i] = (g < QF && j < N) ? a.PhiC[((size_t)c * QF + g) * N + j] : 0.0;
-    }
     for (int i = t; i < 3 * QF; i += SH::THREADS) {
         const int m = i / QF, g = i - m * QF;
         sPinv[m * QF + a.perm[i]] = g;
@@ -911,6 +919,7 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
     if (t < 24) sCanon[t] = a.canon[t];
     __syncthreads();
     const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
+    double* so = sOut + wave * SH::OUT;
 
     for (int64_t tile = (int64_t)blockIdx.x * SH::WAVES + wave; tile < a.ntiles;
          tile += (int64_t)gridDim.x * SH::WAVES) {
@@ -966,7 +975,7 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
                     const bool mine = cl == c;
                     double b[JT];
 #pragma unroll
-                    for (int jt = 0; jt < JT; ++jt) b[jt] = sB[((c * KS + ks) * JT + jt) * 64 + l];
+                    for (int jt = 0; jt < JT; ++jt) b[jt] = a.PhiB[((c * KS + ks) * JT + jt) * 64 + l];
 #pragma unroll
                     for (int v = 0; v < 5; ++v) {
                         const double av = mine ? x[v] : 0.0;
@@ -976,31 +985,31 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
                 }
             }
         }
-        // epilogue: accumulator element r of this lane is element e0 + row(r, kq), test function
-        // 16 jt + (l & 15)
+        // accumulator element r of this lane: tile element row(r, kq), test function 16 jt + (l & 15)
+        wave_sync();  // the previous tile's epilogue reads are done
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t e = e0 + M::row(r, kq);
-            if (e >= a.E) continue;
-            const double det = FUSE ? a.cgeo[e * 10 + 9] : 0.0;
-            const double dte = FUSE ? (a.edt ? a.edt[e] : a.dt) : 0.0;
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int jt = 0; jt < JT; ++jt) {
-                const int j = 16 * jt + (l & 15);
-                if (j >= N) continue;
-                const int64_t i0 = e * NV + 5 * j;
-                if (FUSE) {
-                    const double Mj = det * a.mhat[j];
+                const int j = 16 * jt + row;
+                if (j < N)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) {
-                        const double Rt = a.Rv[i0 + v] + acc[v][jt][r];
-                        const double sv = a.U[i0 + v] + dte * (-Rt / Mj);
-                        a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
-                    }
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) a.R[i0 + v] = a.Rv ? a.Rv[i0 + v] + acc[v][jt][r] : acc[v][jt][r];
-                }
+                    for (int v = 0; v < 5; ++v) so[M::row(r, kq) * NV + 5 * j + v] = acc[v][jt][r];
+            }
+        wave_sync();
+        const int64_t nvalid = min<int64_t>(SH::TE, a.E - e0) * NV;
+        for (int i = l; i < nvalid; i += 64) {
+            const int ee = i / NV, kv = i - ee * NV;
+            const int64_t e = e0 + ee, gi = e0 * NV + i;
+            const double rf = so[i];
+            if (FUSE) {
+                const double Mj = a.cgeo[e * 10 + 9] * a.mhat[kv / 5];
+                const double dte = a.edt ? a.edt[e] : a.dt;
+                const double Rt = a.Rv[gi] + rf;
+                const double sv = a.U[gi] + dte * (-Rt / Mj);
+                a.out[gi] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[gi] + a.b0 * sv;
+            } else {
+                a.R[gi] = a.Rv ? a.Rv[gi] + rf : rf;
             }
         }
     }

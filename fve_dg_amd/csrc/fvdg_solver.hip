@@ -141,6 +141,21 @@ int upload_tables(fvdg_ctx* c) {
         return fail(FVDG_ERR_UNSUPPORTED, "face rule is not invariant under vertex rotation (deviation " +
                                               std::to_string(dev) + ")");
     HIP_TRY(c->tPhiC.upload(PC, c->stream));
+    {
+        // canonical tables as per-lane B operands of k_face_test_mfma: [c][ks][jt][lane] holds
+        // PC[c][g = 4 ks + lane / 16][j = 16 jt + lane % 16], zero outside the table
+        const int KS = (T.Qf + 3) / 4, JT = (T.N + 15) / 16;
+        std::vector<double> PB((size_t)8 * KS * JT * 64, 0.0);
+        for (int cc = 0; cc < 8; ++cc)
+            for (int ks = 0; ks < KS; ++ks)
+                for (int jt = 0; jt < JT; ++jt)
+                    for (int ln = 0; ln < 64; ++ln) {
+                        const int g = 4 * ks + ln / 16, j = 16 * jt + ln % 16;
+                        if (g < T.Qf && j < T.N)
+                            PB[(((size_t)cc * KS + ks) * JT + jt) * 64 + ln] = PC[((size_t)cc * T.Qf + g) * T.N + j];
+                    }
+        HIP_TRY(c->tPhiB.upload(PB, c->stream));
+    }
     HIP_TRY(c->tVA32.upload(std::vector<float>(VA.begin(), VA.end()), c->stream));
     HIP_TRY(c->tGA32.upload(std::vector<float>(GA.begin(), GA.end()), c->stream));
     HIP_TRY(c->tPhiC32.upload(std::vector<float>(PC.begin(), PC.end()), c->stream));
@@ -372,6 +387,7 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.cgeo = c->cgeo.p;
     t.cfaces = c->cfaces.p;
     t.cmeta = c->cmeta.p;
+    t.PhiB = c->tPhiB.p;
     t.fcell = c->fcell.p;
     t.fcombo = c->fcombo.p;
     t.mhat = c->tMhat.p;
