@@ -61,9 +61,9 @@ TRAFFIC = {
 
 
 def test_kernel_name(precision, order):
-    """The FP64 test pass runs on the matrix cores up to P4 (unless FVDG_FACE_TEST=lds), the FP32 one
-    and P5 on the LDS kernel."""
-    mfma = precision == "fp64" and order <= 4 and os.environ.get("FVDG_FACE_TEST") != "lds"
+    """The test pass runs on the matrix cores up to P4 in FP64 and up to P3 in FP32 (unless
+    FVDG_FACE_TEST=lds), otherwise on the LDS kernel."""
+    mfma = order <= (4 if precision == "fp64" else 3) and os.environ.get("FVDG_FACE_TEST") != "lds"
     return "k_face_test_mfma" if mfma else "k_face_test_lds"
 
 

@@ -84,6 +84,7 @@ struct fvdg_ctx {
     DevBuf<uint8_t> tCanon;               // per combo: canonical index * 3 + rotation
     DevBuf<int32_t> tPerm;                // face-rule point permutations of the 3 rotations
     DevBuf<double> tPhiB;                 // canonical tables as MFMA B operands (face test)
+    DevBuf<float> tPhiB32;
     int n_cu = 256;
     // device state: U (state), Ua/Ub (stage ping-pong), R (residual)
     DevBuf<double> U, Ua, Ub, R;

@@ -750,7 +750,7 @@ struct FaceTestArgs {
     const int32_t* perm;
     int64_t ntiles;
     const uint8_t* cmeta;  // E*4: vertex combination of the slot's side | 0x80 on boundary faces
-    const double* PhiB;    // k_face_test_mfma: canonical tables as per-lane MFMA B operands
+    const T* PhiB;         // k_face_test_mfma: canonical tables as per-lane MFMA B operands
 };
 
 template <int ORDER>
@@ -868,7 +868,7 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs<T> a) {
 }
 
 // ------------------------------------------------------------------ face test on the matrix cores
-// k_face_test_mfma<ORDER,FUSE>: the test pass of k_face_test_lds as 16x16x4 FP64 MFMA products.
+// k_face_test_mfma<ORDER,FUSE,T>: the test pass of k_face_test_lds as 16x16x4 MFMA products in T.
 // One wavefront = 16 elements (the A rows); the K index is the canonical face point g' of
 // one canonical table c, the columns are test functions j:
 //   R[e][j][v] += sum_slots sum_g' Fh_slot[pinv_m(g')][v] * PhiC[c][g'][j]
@@ -877,21 +877,27 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs<T> a) {
 // Per slot the tile's elements use few tables (two on a conforming tet mesh: the face opposite
 // vertex s in either orientation), so for each table in use (wave-uniform ballot) the product
 // runs with the A rows of elements on another table set to zero.  The B operands (tables, zero
-// padded to whole 16x4 tiles) are stored per lane, PhiB[c][ks][jt][lane]: one coalesced 512-byte
-// load per product, served by the caches.  The A operands come straight from the face-flux array
-// (each lane's 5 variables of a point are contiguous), with the side's sign.  The products
-// accumulate in registers; the wavefront then writes its 16 x 5N block to LDS in element-major
-// order and the epilogue (R_vol added, fused SSP-RK stage update exactly as k_face_test_lds)
-// reads and writes the state arrays in contiguous runs.
-template <int ORDER>
+// padded to whole 16x4 tiles) are stored per lane, PhiB[c][ks][jt][lane]: one coalesced load per
+// product, served by the caches.  The A operands come straight from the face-flux array (each
+// lane's 5 variables of a point are contiguous), with the side's sign.  The products accumulate
+// in registers; the wavefront then writes its 16 x 5N block to LDS in element-major order and
+// the epilogue (R_vol added, fused SSP-RK stage update exactly as k_face_test_lds) reads and
+// writes the state arrays in contiguous runs.  With T = float the faces' FP64 reference fluxes
+// enter in the epilogue in FP64, sum_slots IC[c][j] * (+-Fref[v]), from a per-tile LDS table.
+template <int ORDER, typename T = double>
 struct FaceTestMfmaShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
     static constexpr int KS = (QF + 3) / 4, JT = (N + 15) / 16;
-    // the epilogue block (16 x 5N doubles per wavefront) sets the LDS per block: from P4 on, one
+    // the epilogue block (16 x 5N values per wavefront) sets the LDS per block: from P4 on, one
     // wavefront per block lets more of them share a compute unit
     static constexpr int WAVES = N > 20 ? 1 : 4, THREADS = 64 * WAVES, TE = 16;
-    static constexpr int OUT = TE * NV;  // doubles of LDS per wavefront
-    static constexpr bool SUPPORTED = WAVES * OUT * 8 <= 96 * 1024;
+    static constexpr int OUT = TE * NV;  // values of LDS per wavefront
+    static constexpr bool SPLIT = !std::is_same<T, double>::value;
+    static constexpr int REF = SPLIT ? TE * 4 * 5 : 1;  // reference fluxes per wavefront
+    // P5's accumulators (4 column tiles x 5 variables) do not fit the registers: P <= 4 in FP64;
+    // in FP32 up to P3 (at P4 the LDS kernel measured faster: 1.2 vs 2.1 ms at 384k tets)
+    static constexpr bool SUPPORTED =
+        N <= (SPLIT ? 20 : 35) && WAVES * (OUT * sizeof(T) + REF * 8 + TE * 4 * 4) <= 96 * 1024;
 };
 
 // LDS ordering between the lanes of one wavefront (the waves of a block run different tile
@@ -902,13 +908,16 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int ORDER, bool FUSE>
-__global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> a) {
-    using SH = FaceTestMfmaShape<ORDER>;
-    using M = Mfma<double>;
-    typedef M::acc_t acc_t;
+template <int ORDER, bool FUSE, typename T>
+__global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<T> a) {
+    using SH = FaceTestMfmaShape<ORDER, T>;
+    using M = Mfma<T>;
+    typedef typename M::acc_t acc_t;
     constexpr int N = SH::N, QF = SH::QF, NV = SH::NV, KS = SH::KS, JT = SH::JT;
-    __shared__ double sOut[SH::WAVES * SH::OUT];
+    constexpr bool SPLIT = SH::SPLIT;
+    __shared__ T sOut[SH::WAVES * SH::OUT];
+    __shared__ double sRef[SH::WAVES * SH::REF];
+    __shared__ int sSlotC[SPLIT ? SH::WAVES * SH::TE * 4 : 1];
     __shared__ int sPinv[3 * QF];
     __shared__ int sCanon[24];
     const int t = threadIdx.x, l = t & 63, wave = t >> 6, row = l & 15, kq = l >> 4;
@@ -919,7 +928,9 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
     if (t < 24) sCanon[t] = a.canon[t];
     __syncthreads();
     const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
-    double* so = sOut + wave * SH::OUT;
+    T* so = sOut + wave * SH::OUT;
+    double* sr = sRef + wave * SH::REF;
+    int* sc = sSlotC + (SPLIT ? wave * SH::TE * 4 : 0);
 
     for (int64_t tile = (int64_t)blockIdx.x * SH::WAVES + wave; tile < a.ntiles;
          tile += (int64_t)gridDim.x * SH::WAVES) {
@@ -928,7 +939,7 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
 #pragma unroll
         for (int v = 0; v < 5; ++v)
 #pragma unroll
-            for (int jt = 0; jt < JT; ++jt) acc[v][jt] = (acc_t){0.0, 0.0, 0.0, 0.0};
+            for (int jt = 0; jt < JT; ++jt) acc[v][jt] = (acc_t){0, 0, 0, 0};
         // the four slots of this lane's A-row element (one 16-byte and one 4-byte load)
         int4 cf4 = make_int4(-1, -1, -1, -1);
         uchar4 cm4 = make_uchar4(0, 0, 0, 0);
@@ -936,12 +947,13 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
             cf4 = *reinterpret_cast<const int4*>(a.cfaces + er * 4);
             cm4 = *reinterpret_cast<const uchar4*>(a.cmeta + er * 4);
         }
+        if (SPLIT) wave_sync();  // the previous tile's epilogue reads of sr / sc are done
 #pragma unroll 1
         for (int s = 0; s < 4; ++s) {
             // this lane's A row: element er, slot s
             int cl = -1, m = 0;
             int64_t f = 0;
-            double sign = 1.0;
+            T sign = 1;
             if (er < a.E) {
                 const int32_t slot = s == 0 ? cf4.x : s == 1 ? cf4.y : s == 2 ? cf4.z : cf4.w;
                 const int meta = s == 0 ? cm4.x : s == 1 ? cm4.y : s == 2 ? cm4.z : cm4.w;
@@ -951,8 +963,14 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
                     cl = cb / 3;
                     m = cb - 3 * cl;
                     f = slot >> 1;
-                    sign = (slot & 1) ? -1.0 : 1.0;
+                    sign = (slot & 1) ? -1 : 1;
                 }
+            }
+            if (SPLIT && kq == 0) {
+                // this slot's FP64 reference flux (signed) and table, for the epilogue
+                sc[row * 4 + s] = cl;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sr[(row * 4 + s) * 5 + v] = cl >= 0 ? (double)sign * a.Fref[f * 5 + v] : 0.0;
             }
             // tables in use by the tile's elements at this slot (wave-uniform)
             unsigned used = 0;
@@ -961,24 +979,24 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const int gc = 4 * ks + kq;
-                double x[5];
+                T x[5];
                 if (cl >= 0 && gc < QF) {
-                    const double* p = a.Fh + (f * QF + sPinv[m * QF + gc]) * 5;
+                    const T* p = a.Fh + (f * QF + sPinv[m * QF + gc]) * 5;
 #pragma unroll
                     for (int v = 0; v < 5; ++v) x[v] = sign * p[v];
                 } else {
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) x[v] = 0.0;
+                    for (int v = 0; v < 5; ++v) x[v] = 0;
                 }
                 for (unsigned u = used; u != 0; u &= u - 1) {
                     const int c = __builtin_ctz(u);
                     const bool mine = cl == c;
-                    double b[JT];
+                    T b[JT];
 #pragma unroll
                     for (int jt = 0; jt < JT; ++jt) b[jt] = a.PhiB[((c * KS + ks) * JT + jt) * 64 + l];
 #pragma unroll
                     for (int v = 0; v < 5; ++v) {
-                        const double av = mine ? x[v] : 0.0;
+                        const T av = mine ? x[v] : (T)0;
 #pragma unroll
                         for (int jt = 0; jt < JT; ++jt) acc[v][jt] = M::mma(av, b[jt], acc[v][jt]);
                     }
@@ -999,11 +1017,19 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<double> 
         wave_sync();
         const int64_t nvalid = min<int64_t>(SH::TE, a.E - e0) * NV;
         for (int i = l; i < nvalid; i += 64) {
-            const int ee = i / NV, kv = i - ee * NV;
+            const int ee = i / NV, kv = i - ee * NV, j = kv / 5;
             const int64_t e = e0 + ee, gi = e0 * NV + i;
-            const double rf = so[i];
+            double rf = (double)so[i];
+            if (SPLIT) {
+                const int v = kv - 5 * j;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int c = sc[ee * 4 + s];
+                    if (c >= 0) rf += a.IC[c * N + j] * sr[(ee * 4 + s) * 5 + v];
+                }
+            }
             if (FUSE) {
-                const double Mj = a.cgeo[e * 10 + 9] * a.mhat[kv / 5];
+                const double Mj = a.cgeo[e * 10 + 9] * a.mhat[j];
                 const double dte = a.edt ? a.edt[e] : a.dt;
                 const double Rt = a.Rv[gi] + rf;
                 const double sv = a.U[gi] + dte * (-Rt / Mj);

@@ -155,6 +155,7 @@ int upload_tables(fvdg_ctx* c) {
                             PB[(((size_t)cc * KS + ks) * JT + jt) * 64 + ln] = PC[((size_t)cc * T.Qf + g) * T.N + j];
                     }
         HIP_TRY(c->tPhiB.upload(PB, c->stream));
+        HIP_TRY(c->tPhiB32.upload(std::vector<float>(PB.begin(), PB.end()), c->stream));
     }
     HIP_TRY(c->tVA32.upload(std::vector<float>(VA.begin(), VA.end()), c->stream));
     HIP_TRY(c->tGA32.upload(std::vector<float>(GA.begin(), GA.end()), c->stream));
@@ -387,7 +388,10 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.cgeo = c->cgeo.p;
     t.cfaces = c->cfaces.p;
     t.cmeta = c->cmeta.p;
-    t.PhiB = c->tPhiB.p;
+    if constexpr (std::is_same<T, double>::value)
+        t.PhiB = c->tPhiB.p;
+    else
+        t.PhiB = c->tPhiB32.p;
     t.fcell = c->fcell.p;
     t.fcombo = c->fcombo.p;
     t.mhat = c->tMhat.p;
@@ -405,11 +409,11 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.PhiC = PhiC;
     t.canon = c->tCanon.p;
     t.perm = c->tPerm.p;
-    if constexpr (std::is_same<T, double>::value && FaceTestMfmaShape<P>::SUPPORTED) {
+    if constexpr (FaceTestMfmaShape<P, T>::SUPPORTED) {
         if (!c->lds_face_test) {
-            using MS = FaceTestMfmaShape<P>;
+            using MS = FaceTestMfmaShape<P, T>;
             t.ntiles = (c->E + MS::TE - 1) / MS::TE;
-            const auto kern = up ? k_face_test_mfma<P, true> : k_face_test_mfma<P, false>;
+            const auto kern = up ? k_face_test_mfma<P, true, T> : k_face_test_mfma<P, false, T>;
             int per_cu = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, MS::THREADS, 0) != hipSuccess || per_cu < 1)
                 per_cu = 1;
