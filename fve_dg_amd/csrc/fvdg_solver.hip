@@ -374,7 +374,17 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
                       c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0};
     constexpr int FB = FaceFluxShape<P, T>::FB;
     f.ntiles = (c->F + FB - 1) / FB;
-    const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * c->grid_mult);
+    // FP64: exactly the resident blocks (measured best at P3: 0.351 ms vs 0.364 with 8 per CU);
+    // FP32: grid_mult blocks per compute unit (more than resident measured faster)
+    int64_t per_cu = c->grid_mult;
+    if constexpr (std::is_same<T, double>::value) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_face_flux_lds<P, T>, FaceFluxShape<P, T>::THREADS, 0) ==
+                hipSuccess &&
+            occ > 0)
+            per_cu = occ;
+    }
+    const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * per_cu);
     Span s1(ms_flux, c->stream);
     hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P, T>::THREADS), 0, c->stream, f);
     s1.stop();
@@ -636,6 +646,7 @@ int fvdg_create(fvdg_ctx** out, int device) {
     c->owner_faces = faces && std::strcmp(faces, "owner") == 0;
     const char* ftest = std::getenv("FVDG_FACE_TEST");
     c->lds_face_test = ftest && std::strcmp(ftest, "lds") == 0;
+    if (const char* gm = std::getenv("FVDG_GRID_MULT")) c->grid_mult = std::max(1, std::atoi(gm));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();
     return FVDG_OK;
