@@ -59,7 +59,13 @@ struct Sys {
 
 int sys_factor(const Sys& S) {
     HIP_TRY(hipMemsetAsync(S.flag, 0, sizeof(int32_t), S.s));
-    if (S.use_inverse()) {
+    if (S.use_inverse() && (S.NB == 5 || S.NB == 20 || S.NB == 50)) {
+        // P0..P2 blocks fit one row per lane: registers, no barriers
+        const auto kern = S.NB == 5 ? k_block_inverse_wave<5> : S.NB == 20 ? k_block_inverse_wave<20>
+                                                                            : k_block_inverse_wave<50>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)S.nrows), dim3(64), 0, S.s, S.A, S.dslot, S.slots, S.Md, S.shift,
+                           S.LU, S.flag);
+    } else if (S.use_inverse()) {
         const size_t lds = sizeof(double) * ((size_t)S.NB * S.NB + S.NB);
         HIP_TRY(hipFuncSetAttribute((const void*)k_block_inverse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_block_inverse, dim3((unsigned)S.nrows), dim3(256), lds, S.s, S.A, S.dslot, S.slots, S.Md,

@@ -507,6 +507,96 @@ __global__ __launch_bounds__(256) void k_block_inverse(const double* __restrict_
     for (int t = tid; t < NB * NB; t += 256) out[t] = sA[(t % NB) * NB + t / NB];
 }
 
+// The same Gauss-Jordan inversion (partial pivoting, row swaps, column swaps undone at the end, the
+// inverse written column-major) for blocks of at most 64 rows, one wavefront per block: each lane
+// holds one row in registers, so a step needs no barrier.  Row swaps are virtual: every lane keeps
+// the position its row has in the swapped order (the pivot is the largest |a_ik| over positions
+// >= k, the smallest position on ties, as in k_block_inverse), and the pivot row reaches every lane
+// through readlane (the pivot lane is wave-uniform).  The entry of column k (runtime) is picked from
+// the registers by compare-and-select.  The arithmetic is that of k_block_inverse.
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_block_inverse_wave(const double* __restrict__ A,
+                                                           const int32_t* __restrict__ diag_slot, int slots,
+                                                           const double* __restrict__ Md, double shift,
+                                                           double* __restrict__ Inv, int32_t* __restrict__ singular) {
+    static_assert(NB <= 64, "one row per lane");
+    __shared__ double sZ[NB * NB];
+    __shared__ int spiv[NB], ssig[NB];
+    const int64_t b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const bool own = lane < NB;
+    const int ds = diag_slot[b];
+    const double* src = ds >= 0 ? A + (b * slots + ds) * (int64_t)NB * NB : nullptr;
+    double a[NB];
+    if (src) {  // wave-uniform; src column-major: entry (lane, j) at j * NB + lane
+        const int rl = own ? lane : 0;  // lanes past the block read row 0 and drop it
+        const double dshift = (Md && own) ? shift * Md[b * NB + lane] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) a[j] = src[j * NB + rl];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) a[j] = own ? a[j] + (lane == j ? dshift : 0.0) : 0.0;
+    } else {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) a[j] = (own && lane == j) ? 1.0 : 0.0;
+    }
+    int pos = own ? lane : NB + lane;  // position of this lane's row in the swapped order
+    bool bad = false;
+#pragma unroll 1
+    for (int k = 0; k < NB; ++k) {
+        double ak = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) ak = (j == k) ? a[j] : ak;
+        double best = (own && pos >= k) ? fabs(ak) : -1.0;
+        int bp = pos, bl = lane;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int op = __shfl_xor(bp, off), ol = __shfl_xor(bl, off);
+            if (ob > best || (ob == best && op < bp)) {
+                best = ob;
+                bp = op;
+                bl = ol;
+            }
+        }
+        const int p = __builtin_amdgcn_readfirstlane(bp), q = __builtin_amdgcn_readfirstlane(bl);
+        bad = bad || !(best > 0.0);
+        if (lane == 0) spiv[k] = p;
+        // the row at position k moves to the pivot's position, the pivot row to position k
+        pos = lane == q ? k : (pos == k ? p : pos);
+        const double pinv = 1.0 / readlane_f64(ak, q);
+        const bool isq = lane == q;
+        const double colk = isq ? pinv : -ak * pinv;  // the new column k
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {  // branch-free: selects only
+            const double kj = readlane_f64(a[j], q) * pinv;
+            const double v = isq ? kj : a[j] - ak * kj;
+            a[j] = (j == k) ? colk : v;
+        }
+    }
+    if (bad && lane == 0) singular[0] = 1;  // benign race: every writer stores 1
+    if (own)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) sZ[pos * NB + j] = a[j];
+    __syncthreads();
+    if (lane == 0) {  // the column swaps undone in reverse order, as one permutation
+        for (int c = 0; c < NB; ++c) ssig[c] = c;
+        for (int k = NB - 1; k >= 0; --k) {
+            const int q = spiv[k], t = ssig[k];
+            ssig[k] = ssig[q];
+            ssig[q] = t;
+        }
+    }
+    __syncthreads();
+    double* out = Inv + b * (int64_t)NB * NB;  // column-major: entry (r, c) at c * NB + r
+    for (int t = lane; t < NB * NB; t += 64) out[t] = sZ[(t % NB) * NB + ssig[t / NB]];
+}
+
 // z = P^{-1} r per block row (forward and back substitution with the pivots of k_block_lu)
 __global__ __launch_bounds__(256) void k_block_lu_solve(const double* __restrict__ LU, const int32_t* __restrict__ piv,
                                                         const double* __restrict__ r, double* __restrict__ z, int NB) {

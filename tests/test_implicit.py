@@ -304,3 +304,27 @@ def test_gpu_backward_euler_picard_step_matches_dense_iteration(solver, oracle):
     s0 = (M * U0)[:, :5].sum(0)
     s1 = (M * solver.download())[:, :5].sum(0)
     assert np.abs(s1 - s0).max() < 1e-10 * np.abs(s0).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 4])
+def test_gpu_block_jacobi_is_the_exact_inverse_of_a_block_diagonal_system(solver, oracle, order):
+    """Volume blocks only (a block-diagonal system): the block-Jacobi preconditioner (register
+    Gauss-Jordan for 5/20/50 unknowns per block, the LDS kernel for 100, LU factors beyond) is
+    the exact inverse, so GMRES stops after one iteration on the dense per-block solution."""
+    print(f"[case] block-Jacobi exact P{order}", flush=True)
+    mesh = F.Mesh.box(2, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=9)
+    U = mesh.project(order, "vortex", VORTEX)
+    solver.set_scheme(order, flux="lf").set_mesh(mesh)
+    solver.upload(U)
+    solver.implicit_assemble(parts=F.PART_CELLS)
+    blk = solver.implicit_blocks()
+    E, NB = U.shape
+    assert np.abs(blk["off"]).max() == 0.0
+    shift = 3.0
+    M = oracle.mass(mesh.export(), order)
+    b = np.random.default_rng(order).standard_normal((E, NB))
+    x, it, rr = solver.implicit_solve(b, shift=shift, tol=1e-12, restart=10, max_iters=10)
+    xs = np.array([np.linalg.solve(blk["diag"][e] + shift * np.diag(M[e]), b[e]) for e in range(E)])
+    assert it == 1 and rr < 1e-12
+    assert rel_err(x, xs) < 1e-11
