@@ -82,15 +82,25 @@ int sys_factor(const Sys& S) {
     return FVDG_OK;
 }
 
+// y = (A + shift diag(Md)) x over `slots` blocks per block row
+void bsr_product(hipStream_t st, const double* A, const int32_t* col, int slots, const double* Md, double shift,
+                 const double* x, double* y, int64_t nrows, int NB) {
+    const dim3 grid(nblk(nrows * NB, 256)), block(256);
+    switch (NB) {
+        case 20: hipLaunchKernelGGL(k_bsr_matvec_nb<20>, grid, block, 0, st, A, col, slots, Md, shift, x, y, nrows); break;
+        case 50: hipLaunchKernelGGL(k_bsr_matvec_nb<50>, grid, block, 0, st, A, col, slots, Md, shift, x, y, nrows); break;
+        case 100: hipLaunchKernelGGL(k_bsr_matvec_nb<100>, grid, block, 0, st, A, col, slots, Md, shift, x, y, nrows); break;
+        default: hipLaunchKernelGGL(k_bsr_matvec, grid, block, 0, st, A, col, slots, Md, shift, x, y, nrows, NB);
+    }
+}
+
 void sys_matvec(const Sys& S, const double* x, double* y) {
-    hipLaunchKernelGGL(k_bsr_matvec, dim3(nblk(S.n(), 256)), dim3(256), 0, S.s, S.A, S.col, S.slots, S.Md, S.shift, x,
-                       y, S.nrows, S.NB);
+    bsr_product(S.s, S.A, S.col, S.slots, S.Md, S.shift, x, y, S.nrows, S.NB);
 }
 
 void sys_precond(const Sys& S, const double* r, double* z) {
     if (S.use_inverse())
-        hipLaunchKernelGGL(k_bsr_matvec, dim3(nblk(S.n(), 256)), dim3(256), 0, S.s, S.LU, S.self, 1, nullptr, 0.0, r,
-                           z, S.nrows, S.NB);
+        bsr_product(S.s, S.LU, S.self, 1, nullptr, 0.0, r, z, S.nrows, S.NB);
     else
         hipLaunchKernelGGL(k_block_lu_solve, dim3((unsigned)S.nrows), dim3(256), sizeof(double) * S.NB, S.s, S.LU,
                            S.piv, r, z, S.NB);

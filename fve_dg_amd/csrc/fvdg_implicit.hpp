@@ -269,10 +269,12 @@ __global__ __launch_bounds__(256) void k_assemble(AssembleArgs a) {
     }
     __syncthreads();
 
-    // B. blocks: one (test j, trial i) pair per thread and round, 25 entries each
+    // B. blocks: one (test j, trial i) pair per thread and round, 25 entries each; consecutive
+    // threads take consecutive test functions j, i.e. rows 5 j + v of the column-major blocks, so
+    // a wavefront's stores of one entry fall in a few cache lines
     double* Ae = a.A + e * kSlots * (int64_t)NB * NB;
     for (int p = tid; p < N * N; p += 256) {
-        const int j = p / N, i = p % N;
+        const int i = p / N, j = p % N;
         double dg[25];
 #pragma unroll
         for (int r = 0; r < 25; ++r) dg[r] = 0.0;
@@ -355,6 +357,32 @@ __global__ void k_bsr_matvec(const double* __restrict__ A, const int32_t* __rest
         const double* blk = A + (b * slots + s) * (int64_t)NB * NB + r;
         const double* xc = x + (int64_t)c * NB;
         for (int k = 0; k < NB; ++k) acc += blk[(int64_t)k * NB] * xc[k];
+    }
+    if (Md) acc += shift * Md[t] * x[t];
+    y[t] = acc;
+}
+
+// k_bsr_matvec with the block size known at compile time: the inner product over a block column
+// is unrolled, so every thread keeps many independent loads in flight
+template <int NB>
+__global__ __launch_bounds__(256) void k_bsr_matvec_nb(const double* __restrict__ A, const int32_t* __restrict__ col,
+                                                       int slots, const double* __restrict__ Md, double shift,
+                                                       const double* __restrict__ x, double* __restrict__ y,
+                                                       int64_t nrows) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows * NB) return;
+    const int64_t b = t / NB;
+    const int r = (int)(t - b * NB);
+    double acc = 0.0;
+    for (int s = 0; s < slots; ++s) {
+        const int32_t c = col[b * slots + s];
+        if (c < 0) continue;
+        const double* blk = A + (b * slots + s) * (int64_t)NB * NB + r;
+        const double* xc = x + (int64_t)c * NB;
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NB; ++k) part[k & 3] += blk[(int64_t)k * NB] * xc[k];
+        acc += (part[0] + part[1]) + (part[2] + part[3]);
     }
     if (Md) acc += shift * Md[t] * x[t];
     y[t] = acc;
