@@ -76,6 +76,8 @@ struct VolShape {
     static constexpr int JT = (N + 15) / 16;  // test-function tiles of the test product
     static constexpr int QPAD = 16 * QT, KPAD = 4 * KS, JPAD = 16 * JT;
     static constexpr int STRIDE = NV | 1;     // odd LDS row stride: spreads banks
+    // waves per SIMD the register allocation is held to: P4 fits two (measured faster than one)
+    static constexpr int MIN_WAVES = N == 35 ? 2 : 1;
 };
 
 // The 16x16x4 matrix-core product in the contraction type T.  Operand lanes are the
@@ -118,7 +120,7 @@ struct VolArgs {
 // FP64.  FP32 rounding then scales with the variation of the state inside the cell,
 // and a uniform state has an FP64-exact residual.
 template <int ORDER, typename T, bool EXTREMA = false>
-__global__ __launch_bounds__(64) void k_vol_mfma(VolArgs<T> a) {
+__global__ __launch_bounds__(64, VolShape<ORDER>::MIN_WAVES) void k_vol_mfma(VolArgs<T> a) {
     using S = VolShape<ORDER>;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
