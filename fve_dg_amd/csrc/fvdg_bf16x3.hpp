@@ -33,6 +33,7 @@ struct Bf3Shape {
     // table sizes in bf16 values per part
     static constexpr int VB = QT * KS * 64 * 8;      // interpolation A operands
     static constexpr int GB = QT * 2 * JT * 64 * 8;  // test A operands (2 K slices per tile)
+    static constexpr int MIN_WAVES = N == 35 ? 2 : 1;  // P4: registers held to 2 waves per SIMD
 };
 
 struct Bf3Args {
@@ -69,11 +70,13 @@ __device__ __forceinline__ f32x4 mma6(const bf16x8 (&a)[3], const bf16x8 (&b)[3]
 }
 
 template <int ORDER>
-__global__ __launch_bounds__(64) void k_vol_bf16x3(Bf3Args a) {
+__global__ __launch_bounds__(64, Bf3Shape<ORDER>::MIN_WAVES) void k_vol_bf16x3(Bf3Args a) {
     using S = Bf3Shape<ORDER>;
     constexpr int N = S::N, Q = S::Q, NV = S::NV, QT = S::QT, KS = S::KS, JT = S::JT, STR = S::STR;
-    __shared__ float sC[16 * STR];
-    __shared__ double sOut[16 * STR];
+    // the tile's coefficients (FP32), later the FP64 result in the same LDS
+    __shared__ double sBuf[16 * STR];
+    float* sC = reinterpret_cast<float*>(sBuf);
+    double* sOut = sBuf;
     const int l = threadIdx.x, col = l & 15, grp = l >> 4;
     const int64_t e0 = (int64_t)blockIdx.x * 16;
     const int64_t base = e0 * NV, limit = a.E * NV;
@@ -189,6 +192,7 @@ __global__ __launch_bounds__(64) void k_vol_bf16x3(Bf3Args a) {
     }
 
     // 6. R = test product + the mean flux's exact part, in FP64, then a coalesced store
+    __syncthreads();  // every lane is done with the coefficients in sBuf
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt)
 #pragma unroll

@@ -128,8 +128,10 @@ __global__ __launch_bounds__(64, VolShape<ORDER>::MIN_WAVES) void k_vol_mfma(Vol
     static_assert(!(SPLIT && EXTREMA), "limiter extremes need the FP64 point values");
     constexpr int N = S::N, Q = S::Q, NV = S::NV, QT = S::QT, KS = S::KS, JT = S::JT;
     constexpr int KPAD = S::KPAD, JPAD = S::JPAD, STR = S::STRIDE;
-    __shared__ T sC[16 * STR];
-    __shared__ double sOut[SPLIT ? 16 * STR : 1];
+    // the tile's coefficients, and (split) later the FP64 result in the same LDS
+    __shared__ double sBuf[16 * STR];
+    T* sC = reinterpret_cast<T*>(sBuf);
+    double* sOut = sBuf;
     const int l = threadIdx.x, col = l & 15, grp = l >> 4;
     const int64_t e0 = (int64_t)blockIdx.x * 16;
     const int64_t base = e0 * NV, limit = a.E * NV;
