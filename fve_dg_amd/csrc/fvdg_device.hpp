@@ -93,7 +93,7 @@ struct fvdg_ctx {
     DevBuf<double> Fref;             // mixed precision: per-face reference flux F*5
     bool simple = false;             // FVDG_KERNELS=simple
     bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
-    bool lds_face_test = false;      // FVDG_FACE_TEST=lds: the LDS/VALU FP64 face test kernel
+    bool lds_face_test = false;      // FVDG_FACE_TEST=lds: the LDS/VALU face test kernel
     DevBuf<double> red;              // reduction partials
     DevBuf<double> edt;              // per-element time step (ensembles); empty: the call's dt
     int64_t n_red = 0;
