@@ -68,6 +68,7 @@ struct fvdg_ctx {
     DevBuf<double> cgeo;        // E*10: A^{-1} (9), |det A|
     DevBuf<int32_t> cfaces;     // E*4: face*2 + side
     DevBuf<uint8_t> cmeta;      // E*4: vertex combination of that side | 0x80 on boundary faces
+    DevBuf<int32_t> cnb;        // E*4: neighbour cell across each local face, -1 on the boundary
     DevBuf<int32_t> fcell;      // F*2
     DevBuf<uint8_t> fcombo;     // F*2
     DevBuf<double> fgeo;        // F*4: normal (3), 2*area

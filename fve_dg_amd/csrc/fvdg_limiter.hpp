@@ -138,8 +138,7 @@ __global__ __launch_bounds__(256) void k_limit_extrema(const double* __restrict_
 
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_limit_apply(double* __restrict__ U, const double* __restrict__ ext,
-                                                     const int32_t* __restrict__ cfaces,
-                                                     const int32_t* __restrict__ fcell, const double* __restrict__ V,
+                                                     const int32_t* __restrict__ cnb, const double* __restrict__ V,
                                                      int64_t E, double gamma, int parts) {
     using S = LimShape<ORDER>;
     constexpr int N = S::N, Q = S::Q, NV = S::NV, LPE = S::LPE, EB = S::EB, ET = S::ET;
@@ -154,9 +153,10 @@ __global__ __launch_bounds__(256) void k_limit_apply(double* __restrict__ U, con
             const int64_t e = e0 + i / 5;
             const int k = i % 5;
             double hi = ext[e * 10 + k], lo = ext[e * 10 + 5 + k];
+            const int4 nb4 = *reinterpret_cast<const int4*>(cnb + e * 4);  // the 4 face neighbours
+#pragma unroll
             for (int f = 0; f < 4; ++f) {
-                const int32_t cf = cfaces[e * 4 + f];
-                const int32_t nb = fcell[(cf >> 1) * 2 + (1 - (cf & 1))];
+                const int32_t nb = f == 0 ? nb4.x : f == 1 ? nb4.y : f == 2 ? nb4.z : nb4.w;
                 if (nb >= 0) {
                     const double h = ext[(int64_t)nb * 10 + k], l = ext[(int64_t)nb * 10 + 5 + k];
                     hi = h > hi ? h : hi;

@@ -541,8 +541,8 @@ void launch_limit_extrema(fvdg_ctx* c, const double* Uprev, int only_avg) {
 template <int P>
 void launch_limit_apply_t(fvdg_ctx* c, double* U, int parts) {
     const dim3 grid(nblocks(c->E, fvdg_lim::LimShape<P>::ET));
-    hipLaunchKernelGGL(fvdg_lim::k_limit_apply<P>, grid, dim3(256), 0, c->stream, U, c->lim_ext.p, c->cfaces.p,
-                       c->fcell.p, c->tV.p, c->E, c->gamma, parts);
+    hipLaunchKernelGGL(fvdg_lim::k_limit_apply<P>, grid, dim3(256), 0, c->stream, U, c->lim_ext.p, c->cnb.p, c->tV.p,
+                       c->E, c->gamma, parts);
 }
 void launch_limit_apply(fvdg_ctx* c, double* U, int parts) {
     switch (c->order) {
@@ -695,6 +695,7 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
     std::vector<double> cgeo(E * 10);
     std::vector<int32_t> cfaces(E * 4);
     std::vector<uint8_t> cmeta(E * 4);
+    std::vector<int32_t> cnb(E * 4);  // neighbour cell across each local face, -1 on the boundary
     std::vector<double> det(E);
     double h_min = INFINITY;
     for (int64_t e = 0; e < E; ++e) {
@@ -742,6 +743,7 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
                 side = has_opp ? 1 : 0;
             }
             cfaces[e * 4 + l] = (int32_t)(2 * f + side);
+            cnb[e * 4 + l] = Fc.is_boundary() ? -1 : (int32_t)Fc.m_neighbor_cells[1 - side];
             cmeta[e * 4 + l] = (uint8_t)(fcombo[2 * f + side] | (Fc.is_boundary() ? 0x80 : 0));
         }
     }
@@ -756,6 +758,7 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
     HIP_TRY(c->cgeo.upload(cgeo, c->stream));
     HIP_TRY(c->cfaces.upload(cfaces, c->stream));
     HIP_TRY(c->cmeta.upload(cmeta, c->stream));
+    HIP_TRY(c->cnb.upload(cnb, c->stream));
     HIP_TRY(c->fcell.upload(fcell, c->stream));
     HIP_TRY(c->fcombo.upload(fcombo, c->stream));
     HIP_TRY(c->fgeo.upload(fgeo, c->stream));
