@@ -100,14 +100,17 @@ def test_autotune_keeps_state_and_results(oracle):
 
 
 def test_mixed_precision_ensemble_tracks_fp64():
-    print("[case] ensemble fp32 vs fp64", flush=True)
+    """FP32 and BF16x3 contractions in a graph-replayed ensemble stay within 1e-6 (relative L2) of
+    the FP64 ensemble after 10 steps, problem by problem."""
+    print("[case] ensemble fp32 / bf16x3 vs fp64", flush=True)
     out = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16x3"):
         _, ens, _ = build(precision=prec)
         ens.set_dt(np.full(len(SPECS), 0.01))
         ens.step(10, graph=True)
         out[prec] = [ens.download(p) for p in range(len(SPECS))]
         ens.close()
-    for p in range(len(SPECS)):
-        a, b = out["fp32"][p], out["fp64"][p]
-        assert np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()) < 1e-6, p
+    for prec in ("fp32", "bf16x3"):
+        for p in range(len(SPECS)):
+            a, b = out[prec][p], out["fp64"][p]
+            assert np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()) < 1e-6, (prec, p)
