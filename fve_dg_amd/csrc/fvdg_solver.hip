@@ -44,6 +44,18 @@ int check_ready(fvdg_ctx* c, bool need_state) {
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// resident blocks per compute unit of a kernel (at least 1), queried once per kernel: the
+// persistent face kernels size their grids with it on every stage
+template <auto Kern, int THREADS>
+int resident_blocks() {
+    static const int n = [] {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, Kern, THREADS, 0) != hipSuccess || occ < 1) occ = 1;
+        return occ;
+    }();
+    return n;
+}
+
 // bf16 bits of a float, rounded to nearest even
 uint16_t bf16_bits(float f) {
     uint32_t u;
@@ -377,13 +389,8 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     // FP64: exactly the resident blocks (measured best at P3: 0.351 ms vs 0.364 with 8 per CU);
     // FP32: grid_mult blocks per compute unit (more than resident measured faster)
     int64_t per_cu = c->grid_mult;
-    if constexpr (std::is_same<T, double>::value) {
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_face_flux_lds<P, T>, FaceFluxShape<P, T>::THREADS, 0) ==
-                hipSuccess &&
-            occ > 0)
-            per_cu = occ;
-    }
+    if constexpr (std::is_same<T, double>::value)
+        per_cu = resident_blocks<k_face_flux_lds<P, T>, FaceFluxShape<P, T>::THREADS>();
     const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * per_cu);
     Span s1(ms_flux, c->stream);
     hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P, T>::THREADS), 0, c->stream, f);
@@ -424,9 +431,8 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
             using MS = FaceTestMfmaShape<P, T>;
             t.ntiles = (c->E + MS::TE - 1) / MS::TE;
             const auto kern = up ? k_face_test_mfma<P, true, T> : k_face_test_mfma<P, false, T>;
-            int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, MS::THREADS, 0) != hipSuccess || per_cu < 1)
-                per_cu = 1;
+            const int per_cu = up ? resident_blocks<k_face_test_mfma<P, true, T>, MS::THREADS>()
+                                  : resident_blocks<k_face_test_mfma<P, false, T>, MS::THREADS>();
             const unsigned g2 = (unsigned)std::min<int64_t>((t.ntiles + MS::WAVES - 1) / MS::WAVES,
                                                             (int64_t)c->n_cu * per_cu);
             Span s2(ms_test, c->stream);
