@@ -23,12 +23,13 @@
 //   last point's values), read off the interpolated point values it already holds.
 //
 // Faces (no atomics anywhere; every output has one fixed summation order):
-//   k_face_flux_lds + k_face_test_lds  two passes (default): each face's traces and
-//     Riemann flux once, then the test functions per element;
-//   k_surf_lds                          one pass, owner-computes: each element
+//   k_face_flux_lds, then k_face_test_mfma   two passes (default): each face's traces and
+//     Riemann flux once, then the test functions per element on the matrix cores (FP64 up
+//     to P4, FP32 up to P3; k_face_test_lds, lanes = (element, test function), elsewhere);
+//   k_surf_lds                                one pass, owner-computes: each element
 //     evaluates its own four faces (both traces and the flux are computed by both
 //     neighbours).
-//   Both finish with the SSP-RK stage update (inverse diagonal mass + Shu-Osher
+//   All finish with the SSP-RK stage update (inverse diagonal mass + Shu-Osher
 //   combination) when used inside the time stepper, so the total residual never goes
 //   to memory.
 #pragma once
