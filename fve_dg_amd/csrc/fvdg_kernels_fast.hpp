@@ -896,7 +896,8 @@ struct FaceTestMfmaShape {
     // the epilogue block (16 x 5N values per wavefront) sets the LDS per block: from P4 on, one
     // wavefront per block lets more of them share a compute unit
     static constexpr int WAVES = N > 20 ? 1 : 4, THREADS = 64 * WAVES, TE = 16;
-    static constexpr int OUT = TE * NV;  // values of LDS per wavefront
+    static constexpr int OUT = TE / 2 * NV;  // values of LDS per wavefront: half a tile at a time
+    static constexpr int MIN_WAVES = N == 20 ? 4 : 3;  // waves per SIMD the registers are held to (P3: 4, measured)
     static constexpr bool SPLIT = !std::is_same<T, double>::value;
     static constexpr int REF = SPLIT ? TE * 4 * 5 : 1;  // reference fluxes per wavefront
     // P5's accumulators (4 column tiles x 5 variables) do not fit the registers: P <= 4 in FP64;
@@ -914,7 +915,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <int ORDER, bool FUSE, typename T>
-__global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<T> a) {
+__global__ __launch_bounds__(256, (FaceTestMfmaShape<ORDER, T>::MIN_WAVES)) void k_face_test_mfma(FaceTestArgs<T> a) {
     using SH = FaceTestMfmaShape<ORDER, T>;
     using M = Mfma<T>;
     typedef typename M::acc_t acc_t;
@@ -1008,39 +1009,47 @@ __global__ __launch_bounds__(256, 3) void k_face_test_mfma(FaceTestArgs<T> a) {
                 }
             }
         }
-        // accumulator element r of this lane: tile element row(r, kq), test function 16 jt + (l & 15)
-        wave_sync();  // the previous tile's epilogue reads are done
+        // accumulator element r of this lane: tile element row(r, kq), test function 16 jt + (l & 15);
+        // the epilogue runs on 8 elements at a time (half the LDS per wavefront)
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            wave_sync();  // the previous half's (or tile's) epilogue reads are done
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < 4; ++r) {
+                const int rw = M::row(r, kq) - 8 * h;
+                if (rw < 0 || rw >= 8) continue;
 #pragma unroll
-            for (int jt = 0; jt < JT; ++jt) {
-                const int j = 16 * jt + row;
-                if (j < N)
+                for (int jt = 0; jt < JT; ++jt) {
+                    const int j = 16 * jt + row;
+                    if (j < N)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) so[M::row(r, kq) * NV + 5 * j + v] = acc[v][jt][r];
-            }
-        wave_sync();
-        const int64_t nvalid = min<int64_t>(SH::TE, a.E - e0) * NV;
-        for (int i = l; i < nvalid; i += 64) {
-            const int ee = i / NV, kv = i - ee * NV, j = kv / 5;
-            const int64_t e = e0 + ee, gi = e0 * NV + i;
-            double rf = (double)so[i];
-            if (SPLIT) {
-                const int v = kv - 5 * j;
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int c = sc[ee * 4 + s];
-                    if (c >= 0) rf += a.IC[c * N + j] * sr[(ee * 4 + s) * 5 + v];
+                        for (int v = 0; v < 5; ++v) so[rw * NV + 5 * j + v] = acc[v][jt][r];
                 }
             }
-            if (FUSE) {
-                const double Mj = a.cgeo[e * 10 + 9] * a.mhat[j];
-                const double dte = a.edt ? a.edt[e] : a.dt;
-                const double Rt = a.Rv[gi] + rf;
-                const double sv = a.U[gi] + dte * (-Rt / Mj);
-                a.out[gi] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[gi] + a.b0 * sv;
-            } else {
-                a.R[gi] = a.Rv ? a.Rv[gi] + rf : rf;
+            wave_sync();
+            const int64_t eh = e0 + 8 * h;
+            const int64_t nvalid = max<int64_t>(0, min<int64_t>(8, a.E - eh)) * NV;
+            for (int i = l; i < nvalid; i += 64) {
+                const int ee = i / NV, kv = i - ee * NV, j = kv / 5;
+                const int64_t e = eh + ee, gi = eh * NV + i;
+                double rf = (double)so[i];
+                if (SPLIT) {
+                    const int v = kv - 5 * j, et = 8 * h + ee;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int c = sc[et * 4 + s];
+                        if (c >= 0) rf += a.IC[c * N + j] * sr[(et * 4 + s) * 5 + v];
+                    }
+                }
+                if (FUSE) {
+                    const double Mj = a.cgeo[e * 10 + 9] * a.mhat[j];
+                    const double dte = a.edt ? a.edt[e] : a.dt;
+                    const double Rt = a.Rv[gi] + rf;
+                    const double sv = a.U[gi] + dte * (-Rt / Mj);
+                    a.out[gi] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[gi] + a.b0 * sv;
+                } else {
+                    a.R[gi] = a.Rv ? a.Rv[gi] + rf : rf;
+                }
             }
         }
     }
