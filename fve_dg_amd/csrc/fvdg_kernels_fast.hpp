@@ -527,7 +527,7 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
 
 // ------------------------------------------------------------------ faces in two passes
 // k_face_flux_lds<ORDER,T>: every face once (face-based, no duplicated work): lanes =
-//   (face, 2 face points); both traces from LDS-staged coefficient blocks and the
+//   (face, face point) (two points at P5 in FP64); both traces from LDS-staged coefficient blocks and the
 //   canonical LDS tables (contraction in T), ghost state on boundary faces, LF/Roe
 //   flux in FP64, scaled by w_g * 2A_f and written to Fh[f][g][v] (type T,
 //   contiguous per face).
@@ -565,17 +565,21 @@ struct FaceFluxArgs {
 template <int ORDER, typename T = double>
 struct FaceFluxShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF;
-    // face points per lane: two in FP64 (the trace products reuse each staged coefficient twice),
-    // one in FP32 (fewer registers per lane pay off there; measured at P3)
-    static constexpr int GP = std::is_same<T, double>::value ? 2 : 1, LPF = (QF + GP - 1) / GP;  // lanes per face
-    // P5's table (8 x 36 x 57 doubles, 131 KB) leaves room for only a small tile; elsewhere 4
-    // wavefronts per block share one copy of the tables (P3: 2 blocks = 8 waves per CU)
-    static constexpr int THREADS = N > 35 ? 64 : 256;
+    static constexpr bool F64 = std::is_same<T, double>::value;
+    // face points per lane: one, except at P5 in FP64 (two: the trace products reuse each
+    // staged coefficient twice, and the 64-thread tile leaves the registers for it)
+    static constexpr int GP = F64 && N > 35 ? 2 : 1, LPF = (QF + GP - 1) / GP;  // lanes per face
+    // P5's table (8 x 36 x 57 doubles, 131 KB) leaves room for only a small tile; elsewhere the
+    // wavefronts of a block share one copy of the tables.  FP64: P4 in one 12-wave block per CU
+    // (147 KB of LDS; 3 waves per SIMD, a few registers spilled), P0-P3 in 4-wave blocks with the
+    // registers held to 3 waves per SIMD at P2 and P3 (P3: 3 blocks, 12 waves per CU; P1 would spill)
+    static constexpr int THREADS = N > 35 ? 64 : (F64 && N == 35 ? 768 : 256);
+    static constexpr int MIN_WAVES = N > 35 || !F64 ? 1 : (N == 35 || N >= 10 ? 3 : 2);
     static constexpr int FB = THREADS / LPF;                // faces per tile
 };
 
 template <int ORDER, typename T>
-__global__ __launch_bounds__(256) void k_face_flux_lds(FaceFluxArgs<T> a) {
+__global__ __launch_bounds__((FaceFluxShape<ORDER, T>::THREADS), (FaceFluxShape<ORDER, T>::MIN_WAVES)) void k_face_flux_lds(FaceFluxArgs<T> a) {
     using SH = FaceFluxShape<ORDER, T>;
     constexpr bool SPLIT = !std::is_same<T, double>::value;
     constexpr int N = SH::N, QF = SH::QF, NV = 5 * N, GP = SH::GP, LPF = SH::LPF, FB = SH::FB;
