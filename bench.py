@@ -53,9 +53,9 @@ TGV = [1.4, 1.0, 1.0, 0.1]  # gamma, rho0, V0, Mach
 # WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), profiles/dg_stage_p3_196k_fp64.md
 TRAFFIC = {
     "vortex_p3_196k": {
-        "volume (k_vol_mfma)": (85017 * 2 + 153600) * 1024,
-        "face flux (k_face_flux_lds)": (174809 * 2 + 368650) * 1024,
-        "face test + update (k_face_test_mfma)": (596251 * 2 + 153600) * 1024,
+        "volume (k_vol_mfma)": (85038 * 2 + 153600) * 1024,
+        "face flux (k_face_flux_lds)": (203502 * 2 + 368640) * 1024,
+        "face test + update (k_face_test_mfma)": (623409 * 2 + 176128) * 1024,
     }
 }
 
