@@ -53,9 +53,9 @@ TGV = [1.4, 1.0, 1.0, 0.1]  # gamma, rho0, V0, Mach
 # WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), profiles/dg_stage_p3_196k_fp64.md
 TRAFFIC = {
     "vortex_p3_196k": {
-        "volume (k_vol_mfma)": (85038 * 2 + 153600) * 1024,
-        "face flux (k_face_flux_lds)": (203502 * 2 + 368640) * 1024,
-        "face test + update (k_face_test_mfma)": (623409 * 2 + 176128) * 1024,
+        "volume (k_vol_mfma)": (85029 * 2 + 153600) * 1024,
+        "face flux (k_face_flux_dpp)": (151621 * 2 + 380525) * 1024,
+        "face test + update (k_face_test_mfma)": (623426 * 2 + 176128) * 1024,
     }
 }
 
@@ -65,6 +65,13 @@ def test_kernel_name(precision, order):
     FVDG_FACE_TEST=lds), otherwise on the LDS kernel."""
     mfma = order <= (4 if precision == "fp64" else 3) and os.environ.get("FVDG_FACE_TEST") != "lds"
     return "k_face_test_mfma" if mfma else "k_face_test_lds"
+
+
+def flux_kernel_name(precision, order):
+    """The FP64 face-flux pass at P2-P4 holds the coefficients in registers and broadcasts them
+    along 16-lane rows (unless FVDG_FACE_FLUX=lds); otherwise it stages them in LDS."""
+    dpp = precision == "fp64" and 2 <= order <= 4 and os.environ.get("FVDG_FACE_FLUX") != "lds"
+    return "k_face_flux_dpp" if dpp else "k_face_flux_lds"
 
 
 def algorithmic(Q, Qf, N, word=8):
@@ -123,8 +130,9 @@ def kernel_roofline(workload, ms, E, Q, Qf, N, precision, order):
     kernels = {
         "volume (k_vol_mfma)": {"ms": ms[1], "achieved": rate(vol_f, ms[1]), "peak": peak_mm, "bound": "mfma",
                                 "unit": "TFLOP/s", "scale": 1e12},
-        "face flux (k_face_flux_lds)": {"ms": ms[2], "achieved": rate(flux_f, ms[2]), "peak": PEAK_FP64,
-                                        "bound": "mfma", "unit": "TFLOP/s", "scale": 1e12},
+        f"face flux ({flux_kernel_name(precision, order)})": {
+            "ms": ms[2], "achieved": rate(flux_f, ms[2]), "peak": PEAK_FP64, "bound": "mfma", "unit": "TFLOP/s",
+            "scale": 1e12},
         f"face test + update ({test_kernel_name(precision, order)})": {
             "ms": ms[3], "achieved": rate(test_b, ms[3]), "peak": PEAK_HBM, "bound": "hbm", "unit": "GB/s",
             "scale": 1e9},

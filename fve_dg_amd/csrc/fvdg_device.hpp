@@ -95,6 +95,7 @@ struct fvdg_ctx {
     bool simple = false;             // FVDG_KERNELS=simple
     bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
     bool lds_face_test = false;      // FVDG_FACE_TEST=lds: the LDS/VALU face test kernel
+    bool lds_face_flux = false;      // FVDG_FACE_FLUX=lds: the LDS-staged face-flux kernel at P2-P4 (FP64)
     DevBuf<double> red;              // reduction partials
     DevBuf<double> edt;              // per-element time step (ensembles); empty: the call's dt
     int64_t n_red = 0;

@@ -35,6 +35,7 @@
 #pragma once
 #include <float.h>
 #include <type_traits>
+#include <utility>
 
 #include "fvdg_kernels.hpp"
 
@@ -728,6 +729,164 @@ __global__ __launch_bounds__((FaceFluxShape<ORDER, T>::THREADS), (FaceFluxShape<
                 T* o = a.Fh + (f * QF + g) * 5;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) o[v] = (T)((fo[i][v] - (SPLIT ? sRef[fl * 5 + v] : 0.0)) * sc);
+            }
+        }
+    }
+}
+
+// k_face_flux_dpp<ORDER>: the FP64 face pass with one face per 16-lane row of a wavefront
+//   (QF <= 32: up to 16 lanes of GP face points).  The two neighbours' coefficient blocks are
+//   not staged in LDS: lane m of the row holds entries m, m + 16, ... of each block in registers
+//   (loaded straight from U, 128 contiguous bytes per row and load), and each entry reaches the
+//   row's lanes through a DPP row broadcast (v_mov_b64_dpp row_newbcast) at the step of the
+//   trace sum that uses it.  LDS holds only the canonical face tables, so a lane reads GP table
+//   values per mode and side instead of the 10 coefficients the LDS kernel re-reads per mode.
+//   Every lane of a row runs the trace sum (a DPP broadcast needs its source lane active);
+//   lanes past the face's points compute a clamped point and discard it.  Flux and output as in
+//   k_face_flux_lds.
+template <int ORDER>
+struct FaceFluxDppShape {
+    static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
+    static constexpr int GP = (QF + 15) / 16, LPF = (QF + GP - 1) / GP;  // points per lane, lanes per face
+    static constexpr int CR = (NV + 15) / 16;                           // coefficient registers per side
+    static constexpr int THREADS = 256, FB = THREADS / 16;              // faces per block and tile
+    // registers held to 3 waves per SIMD up to P3 (44 bytes per lane spilled, measured faster than
+    // 2 waves without spills); P4's tables (56 KB) allow 2 blocks per CU anyway
+    static constexpr int MIN_WAVES = N <= 20 ? 3 : 2;
+    static constexpr bool SUPPORTED = QF <= 32;
+};
+
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int COUNT, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, COUNT>{});
+}
+
+// every lane of a 16-lane row receives the value lane SRC of the row holds (DPP row_newbcast,
+// one 32-bit move per half: the DPP builtins move 32-bit values)
+template <int SRC>
+__device__ __forceinline__ double row_bcast(double x) {
+    const unsigned long u = __builtin_bit_cast(unsigned long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x150 + SRC, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x150 + SRC, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_face_flux_dpp(FaceFluxArgs<double> a) {
+    using SH = FaceFluxDppShape<ORDER>;
+    constexpr int N = SH::N, QF = SH::QF, NV = SH::NV, GP = SH::GP, LPF = SH::LPF, CR = SH::CR, FB = SH::FB;
+    constexpr int NT = SH::THREADS, QS = QF | 1;
+    static_assert(SH::SUPPORTED, "face points exceed one 16-lane row");
+    __shared__ double sPhi[8 * N * QS];  // [table][mode][point], odd point stride
+    __shared__ int sPerm[3 * QF];
+    __shared__ int sCanon[24];
+    const int t = threadIdx.x;
+    for (int i = t; i < 8 * QF * N; i += NT) {
+        const int c = i / (QF * N), g = (i / N) % QF, kk = i % N;
+        sPhi[(c * N + kk) * QS + g] = a.PhiC[i];
+    }
+    for (int i = t; i < 3 * QF; i += NT) sPerm[i] = a.perm[i];
+    if (t < 24) sCanon[t] = a.canon[t];
+    __syncthreads();
+    const int row = t >> 4, m = t & 15;
+
+    for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const int64_t f = tile * FB + row;
+        int kind = 0, L = -1, R = -1, cL = 0, cR = 0;
+        if (f < a.F) {
+            L = a.fcell[2 * f];
+            R = a.fcell[2 * f + 1];
+            cL = sCanon[a.fcombo[2 * f]];
+            cR = sCanon[a.fcombo[2 * f + 1]];
+            if (R >= 0)
+                kind = 1;
+            else if (a.fbc[f] == FVDG_BC_DIRICHLET || a.fbc[f] == FVDG_BC_PSEUDO3DZ)
+                kind = 2;
+        }
+        const int64_t Lc = kind != 0 ? L : -1, Rc = kind == 1 ? R : -1;
+        double cl[CR], cr[CR];
+#pragma unroll
+        for (int r = 0; r < CR; ++r) {
+            const int i = r * 16 + m;
+            cl[r] = (Lc >= 0 && i < NV) ? a.U[Lc * NV + i] : 0.0;
+            cr[r] = (Rc >= 0 && i < NV) ? a.U[Rc * NV + i] : 0.0;
+        }
+        const double* PL = sPhi + (cL / 3) * N * QS;
+        const double* PR = sPhi + (cR / 3) * N * QS;
+        int gl[GP], gr[GP];
+        double UL[GP][5], UR[GP][5], pl[GP], pr[GP];
+#pragma unroll
+        for (int i = 0; i < GP; ++i) {
+            const int g = min(m * GP + i, QF - 1);
+            gl[i] = sPerm[(cL % 3) * QF + g];
+            gr[i] = sPerm[(cR % 3) * QF + g];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) UL[i][v] = UR[i][v] = 0.0;
+        }
+        // trace sums U_side(g)[v] = sum_k phi_side[k](g) c_side[k][v], entry 5k + v of each block
+        // broadcast from lane (5k + v) % 16, register (5k + v) / 16
+        static_for<NV>([&](auto IC) __attribute__((always_inline)) {
+            constexpr int idx = decltype(IC)::value, k = idx / 5, v = idx % 5;
+            if constexpr (v == 0) {
+                // empty asm statements that redefine the accumulators and this mode's table offsets
+                // and coefficient registers in place: the previous mode's products finish above
+                // them and this mode's table reads and broadcasts cannot be hoisted above them
+                // (hoisting all of them spills); no instruction is emitted
+                constexpr int r0 = 5 * k / 16, r1 = (5 * k + 4) / 16;
+#pragma unroll
+                for (int i = 0; i < GP; ++i) {
+                    asm volatile("" : "+v"(gl[i]), "+v"(gr[i]));
+#pragma unroll
+                    for (int w = 0; w < 5; ++w) asm volatile("" : "+v"(UL[i][w]), "+v"(UR[i][w]));
+                }
+                asm volatile("" : "+v"(cl[r0]), "+v"(cr[r0]));
+                if constexpr (r1 != r0) asm volatile("" : "+v"(cl[r1]), "+v"(cr[r1]));
+#pragma unroll
+                for (int i = 0; i < GP; ++i) {
+                    pl[i] = PL[k * QS + gl[i]];
+                    pr[i] = PR[k * QS + gr[i]];
+                }
+            }
+            const double cb = row_bcast<idx % 16>(cl[idx / 16]);
+            const double db = row_bcast<idx % 16>(cr[idx / 16]);
+#pragma unroll
+            for (int i = 0; i < GP; ++i) {
+                UL[i][v] = fma(pl[i], cb, UL[i][v]);
+                UR[i][v] = fma(pr[i], db, UR[i][v]);
+            }
+        });
+        if (f < a.F && m < LPF) {
+            const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
+            const double J = a.fgeo[4 * f + 3];
+#pragma unroll
+            for (int i = 0; i < GP; ++i) {
+                const int g = m * GP + i;
+                if (g >= QF) continue;
+                // outer state: the neighbour's trace, or the ghost state of a Dirichlet (2 U_D - U)
+                // or Pseudo3DZ (mirrored z momentum) face; one Riemann solve per point
+                double fo[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Ub[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Ub[v] = UR[i][v];
+                if (kind == 2) {
+                    if (a.fbc[f] == FVDG_BC_DIRICHLET) {
+                        const double* ud = a.UD + ((int64_t)a.fdir[f] * QF + g) * 5;
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) Ub[v] = ud[v] + (ud[v] - UL[i][v]);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) Ub[v] = UL[i][v];
+                        Ub[3] = -UL[i][3];
+                    }
+                }
+                if (kind != 0) num_flux(a.flux, UL[i], Ub, n, a.gamma, fo);
+                const double sc = a.wf[g] * J;
+                double* o = a.Fh + (f * QF + g) * 5;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) o[v] = fo[v] * sc;
             }
         }
     }

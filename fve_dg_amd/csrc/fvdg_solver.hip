@@ -391,9 +391,21 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     int64_t per_cu = c->grid_mult;
     if constexpr (std::is_same<T, double>::value)
         per_cu = resident_blocks<k_face_flux_lds<P, T>, FaceFluxShape<P, T>::THREADS>();
-    const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * per_cu);
     Span s1(ms_flux, c->stream);
-    hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P, T>::THREADS), 0, c->stream, f);
+    if constexpr (std::is_same<T, double>::value && P >= 2 && FaceFluxDppShape<P>::SUPPORTED) {
+        if (!c->lds_face_flux) {
+            // FP64 P2-P4: one face per 16-lane row, coefficients broadcast from registers
+            constexpr int FBD = FaceFluxDppShape<P>::FB, NTD = FaceFluxDppShape<P>::THREADS;
+            f.ntiles = (c->F + FBD - 1) / FBD;
+            const int64_t res = resident_blocks<k_face_flux_dpp<P>, NTD>();
+            const unsigned gd = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * res);
+            hipLaunchKernelGGL((k_face_flux_dpp<P>), dim3(gd), dim3(NTD), 0, c->stream, f);
+        }
+    }
+    if (!std::is_same<T, double>::value || P < 2 || !FaceFluxDppShape<P>::SUPPORTED || c->lds_face_flux) {
+        const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * per_cu);
+        hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P, T>::THREADS), 0, c->stream, f);
+    }
     s1.stop();
     constexpr int TE = FaceTestShape<P>::TE;
     FaceTestArgs<T> t{};
@@ -652,6 +664,8 @@ int fvdg_create(fvdg_ctx** out, int device) {
     c->owner_faces = faces && std::strcmp(faces, "owner") == 0;
     const char* ftest = std::getenv("FVDG_FACE_TEST");
     c->lds_face_test = ftest && std::strcmp(ftest, "lds") == 0;
+    const char* fflux = std::getenv("FVDG_FACE_FLUX");
+    c->lds_face_flux = fflux && std::strcmp(fflux, "lds") == 0;
     if (const char* gm = std::getenv("FVDG_GRID_MULT")) c->grid_mult = std::max(1, std::atoi(gm));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();

@@ -233,20 +233,25 @@ def test_reference_meshes_divergence_identity(solver, oracle, mesh_name, order, 
 @pytest.mark.parametrize("order,flux,n", [(0, "lf", 4), (1, "lf", 6), (2, "lf", 6), (3, "roe", 8), (4, "roe", 4),
                                           (5, "roe", 4)])
 def test_face_paths_agree(oracle, order, flux, n, monkeypatch):
-    """The two-pass face kernels (default: the test pass on the matrix cores up to P4), the
-    LDS/VALU test pass (FVDG_FACE_TEST=lds) and the one-pass owner-computes kernel
-    (FVDG_FACES=owner), each chosen when a context is created, give the same residual and step."""
+    """The two-pass face kernels (default: the FP64 face-flux pass with register-held coefficients
+    and row broadcasts at P2-P4, the test pass on the matrix cores up to P4), the LDS-staged
+    face-flux pass (FVDG_FACE_FLUX=lds), the LDS/VALU test pass (FVDG_FACE_TEST=lds) and the
+    one-pass owner-computes kernel (FVDG_FACES=owner), each chosen when a context is created,
+    give the same residual and step."""
     print(f"[case] face paths order={order} flux={flux} n={n}", flush=True)
     mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=5)
     U = perturbed_vortex(mesh, order, seed=3)
     out = {}
-    for path in ("default", "owner", "lds"):
+    for path in ("default", "owner", "lds", "lds_flux"):
         monkeypatch.delenv("FVDG_FACES", raising=False)
         monkeypatch.delenv("FVDG_FACE_TEST", raising=False)
+        monkeypatch.delenv("FVDG_FACE_FLUX", raising=False)
         if path == "owner":
             monkeypatch.setenv("FVDG_FACES", path)
         elif path == "lds":
             monkeypatch.setenv("FVDG_FACE_TEST", path)
+        elif path == "lds_flux":
+            monkeypatch.setenv("FVDG_FACE_FLUX", "lds")
         s = F.Solver(0)
         s.set_scheme(order, flux=flux).set_mesh(mesh)
         s.upload(U)
