@@ -23,9 +23,11 @@
 //   last point's values), read off the interpolated point values it already holds.
 //
 // Faces (no atomics anywhere; every output has one fixed summation order):
-//   k_face_flux_lds, then k_face_test_mfma   two passes (default): each face's traces and
-//     Riemann flux once, then the test functions per element on the matrix cores (FP64 up
-//     to P4, FP32 up to P3; k_face_test_lds, lanes = (element, test function), elsewhere);
+//   k_face_flux_dpp or k_face_flux_lds, then k_face_test_mfma   two passes (default): each
+//     face's traces and Riemann flux once (FP64 P2-P4: one face per 16-lane row with the
+//     coefficients broadcast from registers; elsewhere staged in LDS), then the test functions
+//     per element on the matrix cores (FP64 up to P4, FP32 up to P3; k_face_test_lds, lanes =
+//     (element, test function), elsewhere);
 //   k_surf_lds                                one pass, owner-computes: each element
 //     evaluates its own four faces (both traces and the flux are computed by both
 //     neighbours).

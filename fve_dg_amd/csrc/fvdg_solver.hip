@@ -9,9 +9,10 @@
 // stages its 16-element tiles through LDS with full-width coalesced loads.
 //
 // Kernels (fvdg_kernels_fast.hpp): k_vol_mfma (volume, FP64 MFMA), then the faces in
-// two passes: k_face_flux_lds (every face once: traces + Riemann flux) and
-// k_face_test_lds (test functions per element, fused with the SSP-RK3 stage update
-// inside the time stepper).  Environment switches: FVDG_FACES=owner selects the
+// two passes: k_face_flux_dpp / k_face_flux_lds (every face once: traces + Riemann flux) and
+// k_face_test_mfma / k_face_test_lds (test functions per element, fused with the SSP-RK3
+// stage update inside the time stepper).  Environment switches: FVDG_FACE_FLUX=lds and
+// FVDG_FACE_TEST=lds select the LDS forms of the two passes; FVDG_FACES=owner selects the
 // one-pass owner-computes face kernel k_surf_lds; FVDG_KERNELS=simple selects the
 // plain per-point kernels of fvdg_kernels.hpp (the GPU cross-check).
 #include <hip/hip_runtime.h>
