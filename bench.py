@@ -68,9 +68,10 @@ def test_kernel_name(precision, order):
 
 
 def flux_kernel_name(precision, order):
-    """The FP64 face-flux pass at P2-P4 holds the coefficients in registers and broadcasts them
-    along 16-lane rows (unless FVDG_FACE_FLUX=lds); otherwise it stages them in LDS."""
-    dpp = precision == "fp64" and 2 <= order <= 4 and os.environ.get("FVDG_FACE_FLUX") != "lds"
+    """The face-flux pass at P2-P4 holds the coefficients in registers and broadcasts them along
+    16-lane rows, with FP64 traces in every precision (unless FVDG_FACE_FLUX=lds); otherwise it
+    stages them in LDS."""
+    dpp = 2 <= order <= 4 and os.environ.get("FVDG_FACE_FLUX") != "lds"
     return "k_face_flux_dpp" if dpp else "k_face_flux_lds"
 
 

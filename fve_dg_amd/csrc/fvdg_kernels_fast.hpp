@@ -540,7 +540,7 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
 //   and written to R (FP64), or fed to the SSP-RK stage update (FP64).
 // With T = float the parts that cancel between neighbours and against the volume term
 // stay FP64: traces add the mode-0 coefficient in FP64 to an FP32 product over modes
-// >= 1, and each face stores a reference flux Fref (its point-0 flux, FP64, times 2A)
+// >= 1 (k_face_flux_dpp keeps the whole trace in FP64), and each face stores a reference flux Fref (its point-0 flux, FP64, times 2A)
 // plus the FP32 differences Fh[g] = (Fhat_g - Fref) w_g 2A; the test pass adds
 // Fref 2A sum_g w_g phi_j(xi(g)) in FP64 (sum_g w_g phi_j depends only on the
 // canonical face table, table IC).  A uniform flux then cancels exactly in FP64.
@@ -563,6 +563,7 @@ struct FaceFluxArgs {
     T* Fh;         // F*QF*5
     double* Fref;  // F*5 (FP32 path only)
     int64_t ntiles;
+    const double* PhiC64;  // k_face_flux_dpp: the canonical tables in FP64 (its traces are FP64)
 };
 
 template <int ORDER, typename T = double>
@@ -745,7 +746,9 @@ __global__ __launch_bounds__((FaceFluxShape<ORDER, T>::THREADS), (FaceFluxShape<
 //   values per mode and side instead of the 10 coefficients the LDS kernel re-reads per mode.
 //   Every lane of a row runs the trace sum (a DPP broadcast needs its source lane active);
 //   lanes past the face's points compute a clamped point and discard it.  Flux and output as in
-//   k_face_flux_lds.
+//   k_face_flux_lds; with T = float (the mixed-precision stage) the traces and fluxes stay FP64
+//   and the face's reference flux (its point 0, held by lane 0 of the row) reaches the row by one
+//   more broadcast: Fref = F_0 2A in FP64, Fh[g] = (F_g - F_0) w_g 2A in FP32.
 template <int ORDER>
 struct FaceFluxDppShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
@@ -777,8 +780,8 @@ __device__ __forceinline__ double row_bcast(double x) {
     return __builtin_bit_cast(double, ((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
-template <int ORDER>
-__global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_face_flux_dpp(FaceFluxArgs<double> a) {
+template <int ORDER, typename T = double>
+__global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_face_flux_dpp(FaceFluxArgs<T> a) {
     using SH = FaceFluxDppShape<ORDER>;
     constexpr int N = SH::N, QF = SH::QF, NV = SH::NV, GP = SH::GP, LPF = SH::LPF, CR = SH::CR, FB = SH::FB;
     constexpr int NT = SH::THREADS, QS = QF | 1;
@@ -789,12 +792,13 @@ __global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_f
     const int t = threadIdx.x;
     for (int i = t; i < 8 * QF * N; i += NT) {
         const int c = i / (QF * N), g = (i / N) % QF, kk = i % N;
-        sPhi[(c * N + kk) * QS + g] = a.PhiC[i];
+        sPhi[(c * N + kk) * QS + g] = a.PhiC64[i];
     }
     for (int i = t; i < 3 * QF; i += NT) sPerm[i] = a.perm[i];
     if (t < 24) sCanon[t] = a.canon[t];
     __syncthreads();
     const int row = t >> 4, m = t & 15;
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
 
     for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const int64_t f = tile * FB + row;
@@ -861,16 +865,20 @@ __global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_f
                 UR[i][v] = fma(pr[i], db, UR[i][v]);
             }
         });
+        double fo[GP][5];
+#pragma unroll
+        for (int i = 0; i < GP; ++i)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) fo[i][v] = 0.0;
         if (f < a.F && m < LPF) {
             const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
-            const double J = a.fgeo[4 * f + 3];
 #pragma unroll
             for (int i = 0; i < GP; ++i) {
                 const int g = m * GP + i;
                 if (g >= QF) continue;
                 // outer state: the neighbour's trace, or the ghost state of a Dirichlet (2 U_D - U)
                 // or Pseudo3DZ (mirrored z momentum) face; one Riemann solve per point
-                double fo[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Ub[5];
+                double Ub[5];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) Ub[v] = UR[i][v];
                 if (kind == 2) {
@@ -884,11 +892,28 @@ __global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_f
                         Ub[3] = -UL[i][3];
                     }
                 }
-                if (kind != 0) num_flux(a.flux, UL[i], Ub, n, a.gamma, fo);
-                const double sc = a.wf[g] * J;
-                double* o = a.Fh + (f * QF + g) * 5;
+                if (kind != 0) num_flux(a.flux, UL[i], Ub, n, a.gamma, fo[i]);
+            }
+        }
+        // FP32 output: every lane of the row takes point 0's flux from lane 0 (all lanes active)
+        double ref[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if constexpr (SPLIT) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) o[v] = fo[v] * sc;
+            for (int v = 0; v < 5; ++v) ref[v] = row_bcast<0>(fo[0][v]);
+        }
+        if (f < a.F && m < LPF) {
+            const double J = a.fgeo[4 * f + 3];
+            if (SPLIT && m == 0)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) a.Fref[f * 5 + v] = ref[v] * J;
+#pragma unroll
+            for (int i = 0; i < GP; ++i) {
+                const int g = m * GP + i;
+                if (g >= QF) continue;
+                const double sc = a.wf[g] * J;
+                T* o = a.Fh + (f * QF + g) * 5;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) o[v] = (T)((fo[i][v] - ref[v]) * sc);
             }
         }
     }

@@ -384,7 +384,7 @@ template <int P, typename T>
 void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, double* Rout, const StageUpdate* up,
                     const T* PhiC, T* Fh, double* ms_flux, double* ms_test) {
     FaceFluxArgs<T> f{c->F, c->gamma, c->flux, U, c->fcell.p, c->fcombo.p, c->fgeo.p, c->fbc.p, c->fdir.p,
-                      c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0};
+                      c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0, c->tPhiC.p};
     constexpr int FB = FaceFluxShape<P, T>::FB;
     f.ntiles = (c->F + FB - 1) / FB;
     // FP64: exactly the resident blocks (measured best at P3: 0.351 ms vs 0.364 with 8 per CU);
@@ -393,17 +393,18 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     if constexpr (std::is_same<T, double>::value)
         per_cu = resident_blocks<k_face_flux_lds<P, T>, FaceFluxShape<P, T>::THREADS>();
     Span s1(ms_flux, c->stream);
-    if constexpr (std::is_same<T, double>::value && P >= 2 && FaceFluxDppShape<P>::SUPPORTED) {
+    if constexpr (P >= 2 && FaceFluxDppShape<P>::SUPPORTED) {
         if (!c->lds_face_flux) {
-            // FP64 P2-P4: one face per 16-lane row, coefficients broadcast from registers
+            // P2-P4: one face per 16-lane row, coefficients broadcast from registers, FP64 traces
+            // and fluxes (FP32 differences out in the mixed stage)
             constexpr int FBD = FaceFluxDppShape<P>::FB, NTD = FaceFluxDppShape<P>::THREADS;
             f.ntiles = (c->F + FBD - 1) / FBD;
-            const int64_t res = resident_blocks<k_face_flux_dpp<P>, NTD>();
+            const int64_t res = resident_blocks<k_face_flux_dpp<P, T>, NTD>();
             const unsigned gd = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * res);
-            hipLaunchKernelGGL((k_face_flux_dpp<P>), dim3(gd), dim3(NTD), 0, c->stream, f);
+            hipLaunchKernelGGL((k_face_flux_dpp<P, T>), dim3(gd), dim3(NTD), 0, c->stream, f);
         }
     }
-    if (!std::is_same<T, double>::value || P < 2 || !FaceFluxDppShape<P>::SUPPORTED || c->lds_face_flux) {
+    if (P < 2 || !FaceFluxDppShape<P>::SUPPORTED || c->lds_face_flux) {
         const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * per_cu);
         hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P, T>::THREADS), 0, c->stream, f);
     }

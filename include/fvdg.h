@@ -132,7 +132,9 @@ int fvdg_analytic_error(const fvdg_mesh* m, int order, int field, const double* 
 int fvdg_create(fvdg_ctx** out, int device);
 void fvdg_destroy(fvdg_ctx* c);
 /* order 0..5; flux FVDG_FLUX_*; gamma > 1; precision FVDG_PREC_FP64 (the drop-in),
- * FVDG_PREC_FP32 (volume and face contractions in FP32 with FP32 accumulation) or
+ * FVDG_PREC_FP32 (volume and face-test contractions in FP32 with FP32 accumulation; face
+ * traces in FP64 at P2-P4, where that kernel is the faster one, in FP32 elsewhere; face fluxes
+ * stored as FP32 differences from an FP64 reference flux per face) or
  * FVDG_PREC_BF16X3 (volume contractions on the bf16 matrix cores: each FP32 operand split into
  * three bf16 parts, six part products accumulated in FP32; face contractions as FP32).  In both
  * mixed forms the pointwise and Riemann fluxes, stage update, state and conservation sums stay
