@@ -54,8 +54,8 @@ TGV = [1.4, 1.0, 1.0, 0.1]  # gamma, rho0, V0, Mach
 TRAFFIC = {
     "vortex_p3_196k": {
         "volume (k_vol_mfma)": (85029 * 2 + 153600) * 1024,
-        "face flux (k_face_flux_dpp)": (151621 * 2 + 380525) * 1024,
-        "face test + update (k_face_test_mfma)": (623426 * 2 + 176128) * 1024,
+        "face flux (k_face_flux_dpp)": (151599 * 2 + 376357) * 1024,
+        "face test + update (k_face_test_mfma)": (623421 * 2 + 176128) * 1024,
     }
 }
 
