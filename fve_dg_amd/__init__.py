@@ -77,6 +77,7 @@ def lib():
         _sig(L, "fvdg_solution_read", [_vp, C.c_int, C.c_char_p, _d])
         _sig(L, "fvdg_project_analytic", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d])
         _sig(L, "fvdg_analytic_error", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d, _d])
+        _sig(L, "fvdg_device_count", [C.POINTER(C.c_int)])
         _sig(L, "fvdg_create", [C.POINTER(_vp), C.c_int])
         _sig(L, "fvdg_destroy", [_vp], None)
         _sig(L, "fvdg_set_scheme", [_vp, C.c_int, C.c_int, C.c_double, C.c_int])
@@ -123,14 +124,9 @@ def lib():
 
 
 def gpu_available():
-    """True when the HIP runtime sees a device (asks torch, which counts devices without
-    initialising one)."""
-    try:
-        import torch
-
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return False
+    """True when the HIP runtime sees a device (fvdg_device_count of the C ABI)."""
+    n = C.c_int(0)
+    return lib().fvdg_device_count(C.byref(n)) == 0 and n.value > 0
 
 
 def _check(status):

@@ -97,6 +97,8 @@ struct fvdg_ctx {
     bool lds_face_test = false;      // FVDG_FACE_TEST=lds: the LDS/VALU face test kernel
     bool lds_face_flux = false;      // FVDG_FACE_FLUX=lds: the LDS-staged face-flux kernel at P2-P4 (FP64)
     DevBuf<double> red;              // reduction partials
+    DevBuf<int32_t> bad;             // 1 word: set by the stage update when an output is non-finite or a
+                                     // mean density <= 0 (cleared and read by fvdg_ssprk3_step/_stage)
     DevBuf<double> edt;              // per-element time step (ensembles); empty: the call's dt
     int64_t n_red = 0;
     // limiter (PositiveLimiter): per-element extremes of the previous state, staging of

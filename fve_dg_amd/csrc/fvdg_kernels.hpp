@@ -54,6 +54,15 @@ __device__ __forceinline__ int64_t cidx(int64_t e, int kv, int NV) { return e * 
 // quotient is then within one rounding of the reference's).
 __device__ __forceinline__ double pos_eps(double x) { return fmax(x, 1e-16); }
 
+// maximum that keeps a NaN operand (fmax drops it)
+__device__ __forceinline__ double max_keep_nan(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+
+// stage-output check of the SSP-RK update: a non-finite coefficient, or a non-positive mean
+// density (kv == 0 is mode 0 of rho; phi_0 = 1, so that coefficient is the cell mean)
+__device__ __forceinline__ bool stage_value_bad(double x, int kv) {
+    return !(x - x == 0.0) || (kv == 0 && !(x > 0.0));
+}
+
 struct Prim {
     double rho, ri, u, v, w, e, p;  // ri = 1/rho
 };
@@ -106,6 +115,13 @@ __device__ __forceinline__ double euler_wave_speed_prim(const Prim& q, double ga
 
 __device__ __forceinline__ double euler_wave_speed(const double U[5], double gamma) {
     return euler_wave_speed_prim(euler_prim(U, gamma), gamma);
+}
+
+// |u| + c for the time-step reductions: NaN when any conserved value is not finite (the clamps of
+// euler_prim would otherwise turn a NaN energy into a finite sound speed)
+__device__ __forceinline__ double wave_speed_checked(const double U[5], double gamma) {
+    const double s = U[0] + U[1] + U[2] + U[3] + U[4];
+    return (s - s == 0.0) ? euler_wave_speed(U, gamma) : __builtin_nan("");
 }
 
 // Lax-Friedrichs: 0.5 (F(U_L) n + F(U_R) n + lambda (U_L - U_R)), lambda = max(|u|+c)
@@ -319,7 +335,7 @@ __global__ void k_face_test(KernelArgs a) {
 __global__ void k_rk_update(const double* __restrict__ U0, const double* __restrict__ Us,
                             const double* __restrict__ R, double* __restrict__ out, const double* __restrict__ cgeo,
                             const double* __restrict__ mhat, int64_t E, int N, double a0, double b0, double dt,
-                            const double* __restrict__ edt) {
+                            const double* __restrict__ edt, int* __restrict__ bad) {
     const int NV = 5 * N;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E * NV) return;
@@ -328,7 +344,9 @@ __global__ void k_rk_update(const double* __restrict__ U0, const double* __restr
     const double M = cgeo[e * 10 + 9] * mhat[kv / 5];
     const double L = -R[i] / M;
     const double s = Us[i] + (edt ? edt[e] : dt) * L;
-    out[i] = (a0 == 0.0) ? b0 * s : a0 * U0[i] + b0 * s;
+    const double o = (a0 == 0.0) ? b0 * s : a0 * U0[i] + b0 * s;
+    out[i] = o;
+    if (bad && stage_value_bad(o, kv)) bad[0] = 1;
 }
 #endif
 
@@ -343,16 +361,16 @@ __global__ void k_max_wave_speed(KernelArgs a, double* __restrict__ part) {
         const int g = (int)(t - f * a.Qf);
         double U[5];
         face_trace(a, a.fcell[2 * f], a.fcombo[2 * f], g, U);
-        m = euler_wave_speed(U, a.gamma);
+        m = wave_speed_checked(U, a.gamma);
         if (a.fcell[2 * f + 1] >= 0) {
             face_trace(a, a.fcell[2 * f + 1], a.fcombo[2 * f + 1], g, U);
-            m = fmax(m, euler_wave_speed(U, a.gamma));
+            m = max_keep_nan(m, wave_speed_checked(U, a.gamma));
         }
     }
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = max_keep_nan(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
@@ -368,10 +386,10 @@ __global__ void k_face_wave_speed(KernelArgs a, double* __restrict__ out) {
     for (int g = 0; g < a.Qf; ++g) {
         double U[5];
         face_trace(a, a.fcell[2 * f], a.fcombo[2 * f], g, U);
-        m = fmax(m, euler_wave_speed(U, a.gamma));
+        m = max_keep_nan(m, wave_speed_checked(U, a.gamma));
         if (a.fcell[2 * f + 1] >= 0) {
             face_trace(a, a.fcell[2 * f + 1], a.fcombo[2 * f + 1], g, U);
-            m = fmax(m, euler_wave_speed(U, a.gamma));
+            m = max_keep_nan(m, wave_speed_checked(U, a.gamma));
         }
     }
     out[f] = m;

@@ -306,6 +306,7 @@ struct SurfArgs {
     const double* mhat;
     double a0, b0, dt;
     const double* edt;
+    int* bad;  // set to 1 when a stage output is non-finite or a mean density <= 0 (nullable)
 };
 
 // ------------------------------------------------------------------ surface, persistent
@@ -514,12 +515,16 @@ __global__ __launch_bounds__(256) void k_surf_lds(SurfLdsArgs A) {
             const int64_t i0 = e * NV + 5 * j;
             if (FUSE) {
                 const double M = a.cgeo[e * 10 + 9] * a.mhat[j];
+                bool badl = false;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     const double Rt = a.R[i0 + v] + r[it][v];
                     const double sv = a.U[i0 + v] + (a.edt ? a.edt[e] : a.dt) * (-Rt / M);
-                    a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
+                    const double o = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
+                    a.out[i0 + v] = o;
+                    badl |= stage_value_bad(o, 5 * j + v);
                 }
+                if (badl && a.bad) a.bad[0] = 1;
             } else {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) a.R[i0 + v] = a.accumulate ? a.R[i0 + v] + r[it][v] : r[it][v];
@@ -946,6 +951,7 @@ struct FaceTestArgs {
     int64_t ntiles;
     const uint8_t* cmeta;  // E*4: vertex combination of the slot's side | 0x80 on boundary faces
     const T* PhiB;         // k_face_test_mfma: canonical tables as per-lane MFMA B operands
+    int* bad;              // FUSE: set to 1 when an output is non-finite or a mean density <= 0 (nullable)
 };
 
 template <int ORDER>
@@ -1045,12 +1051,16 @@ __global__ __launch_bounds__(256) void k_face_test_lds(FaceTestArgs<T> a) {
             const int64_t i0 = e * NV + 5 * j;
             if (FUSE) {
                 const double M = a.cgeo[e * 10 + 9] * a.mhat[j];
+                bool badl = false;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     const double Rt = a.Rv[i0 + v] + (rc[v] + (double)r[v]);
                     const double sv = a.U[i0 + v] + (a.edt ? a.edt[e] : a.dt) * (-Rt / M);
-                    a.out[i0 + v] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
+                    const double o = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[i0 + v] + a.b0 * sv;
+                    a.out[i0 + v] = o;
+                    badl |= stage_value_bad(o, 5 * j + v);
                 }
+                if (badl && a.bad) a.bad[0] = 1;
             } else {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
@@ -1128,6 +1138,7 @@ __global__ __launch_bounds__(256, (FaceTestMfmaShape<ORDER, T>::MIN_WAVES)) void
     double* sr = sRef + wave * SH::REF;
     int* sc = sSlotC + (SPLIT ? wave * SH::TE * 4 : 0);
 
+    bool badl = false;  // FUSE: a stage output of this lane failed stage_value_bad
     for (int64_t tile = (int64_t)blockIdx.x * SH::WAVES + wave; tile < a.ntiles;
          tile += (int64_t)gridDim.x * SH::WAVES) {
         const int64_t e0 = tile * SH::TE, er = e0 + row;
@@ -1236,11 +1247,14 @@ __global__ __launch_bounds__(256, (FaceTestMfmaShape<ORDER, T>::MIN_WAVES)) void
                     const double dte = a.edt ? a.edt[e] : a.dt;
                     const double Rt = a.Rv[gi] + rf;
                     const double sv = a.U[gi] + dte * (-Rt / Mj);
-                    a.out[gi] = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[gi] + a.b0 * sv;
+                    const double o = (a.a0 == 0.0) ? a.b0 * sv : a.a0 * a.U0[gi] + a.b0 * sv;
+                    a.out[gi] = o;
+                    badl |= stage_value_bad(o, kv);
                 } else {
                     a.R[gi] = a.Rv ? a.Rv[gi] + rf : rf;
                 }
             }
         }
     }
+    if (FUSE && badl && a.bad) a.bad[0] = 1;
 }

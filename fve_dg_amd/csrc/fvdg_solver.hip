@@ -216,6 +216,8 @@ int alloc_state(fvdg_ctx* c) {
     HIP_TRY(c->R.alloc(ndof));
     for (DevBuf<double>* b : {&c->U, &c->Ua, &c->Ub, &c->R})
         HIP_TRY(hipMemsetAsync(b->p, 0, ndof * sizeof(double), c->stream));
+    HIP_TRY(c->bad.alloc(1));
+    HIP_TRY(hipMemsetAsync(c->bad.p, 0, sizeof(int32_t), c->stream));
     if (c->simple) HIP_TRY(c->Fq.alloc((size_t)c->E * c->tab.Q * 15));
     int rc = precision_buffers(c);
     if (rc) return rc;
@@ -294,6 +296,7 @@ SurfArgs surf_args(fvdg_ctx* c, const double* U, double* R, int parts) {
     a.mhat = c->tMhat.p;
     a.a0 = a.b0 = a.dt = 0.0;
     a.edt = c->edt.n ? c->edt.p : nullptr;
+    a.bad = c->bad.p;
     return a;
 }
 
@@ -427,6 +430,7 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     t.fcombo = c->fcombo.p;
     t.mhat = c->tMhat.p;
     t.edt = c->edt.n ? c->edt.p : nullptr;
+    t.bad = c->bad.p;
     if (up) {
         t.U0 = up->U0;
         t.out = up->out;
@@ -596,7 +600,7 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
             const int64_t n = c->E * 5 * c->tab.N;
             hipLaunchKernelGGL(k_rk_update, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, c->U.p, src[s], c->R.p,
                                dst[s], c->cgeo.p, c->tMhat.p, c->E, c->tab.N, A[s], B[s], dt,
-                               c->edt.n ? c->edt.p : nullptr);
+                               c->edt.n ? c->edt.p : nullptr, c->bad.p);
             su.stop();
         } else {
             Span sv(ms ? &ms[1] : nullptr, c->stream);
@@ -615,6 +619,18 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
     return FVDG_OK;
 }
 
+
+// Waits for the stream, then reports the stage-output flag the update kernels set
+// (FVDG_ERR_NONFINITE; the state is left as computed).
+int finish_checked(fvdg_ctx* c) {
+    int32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, c->bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (flag)
+        return fail(FVDG_ERR_NONFINITE,
+                    "an SSP-RK3 stage produced a non-finite coefficient or a non-positive mean density");
+    return FVDG_OK;
+}
 
 }  // namespace
 
@@ -648,6 +664,14 @@ int fvdg::face_wave_speeds(fvdg_ctx* c, std::vector<double>& out) {
 
 // ================================================================ C ABI (device part)
 extern "C" {
+
+int fvdg_device_count(int* n) {
+    if (!n) return fail(FVDG_ERR_ARG, "null output");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) count = 0;  // no driver or no device: none
+    *n = count;
+    return FVDG_OK;
+}
 
 int fvdg_create(fvdg_ctx** out, int device) {
     if (!out) return fail(FVDG_ERR_ARG, "null output pointer");
@@ -886,12 +910,12 @@ int fvdg_ssprk3_step(fvdg_ctx* c, double dt, int n_steps) {
     if (rc) return rc;
     if (n_steps < 0 || !(dt >= 0)) return fail(FVDG_ERR_ARG, "bad dt or step count");
     HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemsetAsync(c->bad.p, 0, sizeof(int32_t), c->stream));
     for (int i = 0; i < n_steps; ++i) {
         rc = ssprk3_once(c, dt, nullptr);
         if (rc) return rc;
     }
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return FVDG_OK;
+    return finish_checked(c);
 }
 
 int fvdg_ssprk3_stage(fvdg_ctx* c, double dt, int stage) {
@@ -899,10 +923,10 @@ int fvdg_ssprk3_stage(fvdg_ctx* c, double dt, int stage) {
     if (rc) return rc;
     if (stage < 0 || stage > 2 || !(dt >= 0)) return fail(FVDG_ERR_ARG, "bad dt or stage index");
     HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemsetAsync(c->bad.p, 0, sizeof(int32_t), c->stream));
     rc = ssprk3_stage(c, dt, stage, nullptr);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return FVDG_OK;
+    return finish_checked(c);
 }
 
 int fvdg_stable_dt(fvdg_ctx* c, double cfl, double* dt) {
@@ -938,7 +962,7 @@ int fvdg_max_wave_speed(fvdg_ctx* c, double* out) {
     HIP_TRY(hipMemcpyAsync(part.data(), c->red.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     double m = 0;
-    for (double v : part) m = std::max(m, v);
+    for (double v : part) m = (v != v || v > m) ? v : m;  // a NaN partial stays NaN
     *out = m;
     return FVDG_OK;
 }
@@ -1041,19 +1065,12 @@ int fvdg_flow_diagnostics(fvdg_ctx* c, double out[8]) {
     return FVDG_OK;
 }
 
-int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms) {
-    int rc = check_ready(c, true);
-    if (rc) return rc;
-    if (!ms || n_stages <= 0 || n_stages % 3) return fail(FVDG_ERR_ARG, "n_stages must be a positive multiple of 3");
-    HIP_TRY(hipSetDevice(c->device));
-    for (int i = 0; i < 4; ++i) ms[i] = 0;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+// fvdg_time_stages with its two events (created and destroyed by the caller)
+static int time_stages_with(fvdg_ctx* c, double dt, int n_stages, double* ms, hipEvent_t e0, hipEvent_t e1) {
     // pass 1: whole stages, no per-kernel events inside (the throughput figure)
     HIP_TRY(hipEventRecord(e0, c->stream));
     for (int s = 0; s < n_stages / 3; ++s) {
-        rc = ssprk3_once(c, dt, nullptr);
+        const int rc = ssprk3_once(c, dt, nullptr);
         if (rc) return rc;
     }
     HIP_TRY(hipEventRecord(e1, c->stream));
@@ -1064,13 +1081,29 @@ int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms) {
     // pass 2: per-kernel split (events between kernels)
     double split[4] = {0, 0, 0, 0};
     for (int s = 0; s < n_stages / 3; ++s) {
-        rc = ssprk3_once(c, dt, split);
+        const int rc = ssprk3_once(c, dt, split);
         if (rc) return rc;
     }
     for (int i = 1; i < 4; ++i) ms[i] = split[i] / n_stages;
+    return FVDG_OK;
+}
+
+int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms) {
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    if (!ms || n_stages <= 0 || n_stages % 3) return fail(FVDG_ERR_ARG, "n_stages must be a positive multiple of 3");
+    HIP_TRY(hipSetDevice(c->device));
+    for (int i = 0; i < 4; ++i) ms[i] = 0;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(FVDG_ERR_HIP, "hipEventCreate failed");
+    }
+    rc = time_stages_with(c, dt, n_stages, ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return FVDG_OK;
+    return rc;
 }
 
 }  // extern "C"

@@ -63,8 +63,9 @@ private:
     std::unique_ptr<fvdg_mesh, Del> h_;
 };
 
-// One device context with a scheme and a mesh.  Rebinding happens when the mesh
-// object, its size or the scheme changes (bind() is cheap otherwise).
+// One device context with a scheme and a mesh.  A scheme change calls only fvdg_set_scheme
+// (which keeps an uploaded state unless the order changes); the mesh is rebound when the mesh
+// object, its sizes or its boundary types change (bind() is cheap otherwise).
 class Session {
 public:
     explicit Session(int device = 0) {
@@ -82,7 +83,7 @@ public:
             gamma_ = gamma;
             precision_ = precision;
         }
-        if (scheme_changed || &cm != mesh_key_ || cm.m_cells.size() != n_cells_ || cm.m_faces.size() != n_faces_ ||
+        if (&cm != mesh_key_ || cm.m_cells.size() != n_cells_ || cm.m_faces.size() != n_faces_ ||
             !same_boundary_types(cm)) {
             mesh_ = std::make_unique<MeshHandle>(cm);
             check(fvdg_set_mesh(ctx_.get(), mesh_->get()), "fvdg_set_mesh");

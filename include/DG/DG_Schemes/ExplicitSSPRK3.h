@@ -27,7 +27,8 @@ public:
     // `mesh` must outlive the stepper.
     explicit ExplicitSSPRK3(const ComputingMesh& mesh) : mesh_(mesh) {}
 
-    // the residual object (Riemann solver, precision, boundary function); configure before set_solution
+    // the residual object (Riemann solver, precision, boundary function); changing the Riemann
+    // solver or the precision after set_solution keeps the uploaded state
     Convection& convection() { return conv_; }
 
     void set_solution(const LongVector<5 * N>& U) {

@@ -3,7 +3,7 @@
 // Names and selector mapping follow the reference's
 // include/GaussQuadrature/GaussLegendre.h:22-111 and AutoQuadSelectorTri.h:1-22;
 // the point/weight doubles come from quad_rules.inc (generated, see
-// gen_quad_rules.py).  Tet weights sum to 1/6, triangle weights to 1/2.
+// tests/golden/gen_quad_rules.py).  Tet weights sum to 1/6, triangle weights to 1/2.
 #pragma once
 #include "base/Type.h"
 #include "GaussQuadrature/quad_rules.inc"

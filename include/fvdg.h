@@ -129,6 +129,9 @@ int fvdg_analytic_error(const fvdg_mesh* m, int order, int field, const double* 
                         const double* U, double err[5]);
 
 /* ---------------------------- device solver ---------------------------- */
+/* number of GPUs the HIP runtime sees (0 without a driver or device); no reference counterpart:
+ * lets callers choose a device without another GPU library */
+int fvdg_device_count(int* n);
 int fvdg_create(fvdg_ctx** out, int device);
 void fvdg_destroy(fvdg_ctx* c);
 /* order 0..5; flux FVDG_FLUX_*; gamma > 1; precision FVDG_PREC_FP64 (the drop-in),
@@ -155,17 +158,22 @@ int fvdg_set_dirichlet_states(fvdg_ctx* c, const double* UD);
 int fvdg_residual(fvdg_ctx* c, double t, int parts, double* R);
 /* Diagonal mass M[e*5N+5k+v] = |det A_e| * sum_q w_q phi_k(xi_q)^2. */
 int fvdg_mass_diagonal(fvdg_ctx* c, double* M);
-/* n_steps SSP-RK3 steps of dU/dt = -M^{-1} R(U) on the device state. */
+/* n_steps SSP-RK3 steps of dU/dt = -M^{-1} R(U) on the device state.  Returns
+ * FVDG_ERR_NONFINITE when a stage wrote a non-finite coefficient or a non-positive mean density
+ * (mode 0 of rho); the steps still run to the end and the state is left as computed.  (The
+ * reference has no such check; its drivers stop on a non-positive time step,
+ * examples/Sedov/SedovBatch.cpp:200.) */
 int fvdg_ssprk3_step(fvdg_ctx* c, double dt, int n_steps);
 /* stage 0, 1 or 2 of one Shu-Osher SSP-RK3 step on the device state (U^n -> U1 -> U2 -> U^{n+1});
  * calling 0, 1, 2 in order equals one fvdg_ssprk3_step.  Between stages the caller may
  * update the Dirichlet states to the stage times t, t + dt, t + dt/2. */
-int fvdg_ssprk3_stage(fvdg_ctx* c, double dt, int stage);
+int fvdg_ssprk3_stage(fvdg_ctx* c, double dt, int stage);  /* same FVDG_ERR_NONFINITE check */
 /* dt = cfl * h_min / ((2p+1) * lambda_max), h = 3 * volume / (sum of the cell's face areas),
  * lambda_max = max over face points of max_{L,R}(|u| + c) of the device state
  * (examples/Sedov/SedovBatch.cpp:174-199 form, with the tet inradius as h) */
 int fvdg_stable_dt(fvdg_ctx* c, double cfl, double* dt);
-/* max over all face quadrature points of max(|u|+c) of the two traces */
+/* max over all face quadrature points of max(|u|+c) of the two traces; NaN when any trace is
+ * not finite (fvdg_stable_dt then returns FVDG_ERR_NONFINITE) */
 int fvdg_max_wave_speed(fvdg_ctx* c, double* out);
 /* sum over elements of |det A_e| * U_{e,k=0,v} / 6 = integral of each conserved variable */
 int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);

@@ -147,8 +147,8 @@ void periodic_case(Writer& w, int flux) {
     conv.set_riemann_solver(flux);
     const auto R = conv.eval(cm, U);
     ExplicitSSPRK3<P, MonatomicFluxC> rk(cm);
-    rk.convection().set_riemann_solver(flux);
     rk.set_solution(U);
+    rk.convection().set_riemann_solver(flux);  // a scheme change after the upload keeps the state
     const Scalar dt = 0.01;
     rk.step(0.0, dt);
     const auto U1 = rk.solution();

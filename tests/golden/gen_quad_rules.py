@@ -21,8 +21,9 @@ Selector mapping (``include/GaussQuadrature/GaussLegendre.h:84-111``,
   4      Degree9Points57   Degree10Points24
   5      Degree11Points94  Degree13Points36
 
-Run in a container that holds the reference tree:
-    python3 fve_dg_amd/csrc/gen_quad_rules.py /root/reference
+Run from the repository root in a container that holds the reference tree; it writes
+``include/GaussQuadrature/quad_rules.inc``:
+    python3 tests/golden/gen_quad_rules.py /root/reference
 """
 import re
 import sys
@@ -153,7 +154,7 @@ def find_rule(root, files, name, dim):
 
 def main():
     ref = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
-    out = Path(__file__).with_name("quad_rules.inc")
+    out = Path(__file__).resolve().parents[2] / "include" / "GaussQuadrature" / "quad_rules.inc"
     lines = [
         "/* Generated by gen_quad_rules.py -- do not edit.  Usable from C and C++:",
         " * each rule is <TAG>_N, <TAG>_PTS (brace list of points), <TAG>_W (weights).",
