@@ -42,13 +42,17 @@ def test_fp32_residual_close_to_oracle(solver, oracle, order, flux, prec):
 
 
 @pytest.mark.parametrize("prec", MIXED)
-def test_fp32_parts_and_boundaries(solver, oracle, prec):
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_fp32_parts_and_boundaries(solver, oracle, prec, order):
+    """Dirichlet (x/y) and Pseudo3DZ (z) faces in the mixed path at P2-P4 (the orders whose
+    face-flux pass holds the coefficients in registers), each residual part against the oracle."""
+    print(f"[case] {prec} boundaries P{order}", flush=True)
     mesh = F.Mesh.box((4, 4, 2), ub=(10, 10, 2.5), split=5)
     mesh.tag_pseudo3dz()
     bc = mesh.boundary_types()
     params = [1.4, 5.0, 4.0, 4.5, 1.0, 0.5, 0.0, 0.0]
-    U = mesh.project(3, "vortex", params)
-    solver.set_scheme(3, flux="roe", precision=prec).set_mesh(mesh)
+    U = mesh.project(order, "vortex", params)
+    solver.set_scheme(order, flux="roe", precision=prec).set_mesh(mesh)
     solver.upload(U)
     _, xyz = solver.dirichlet_points()
     UD = np.array([[oracle.vortex_state(params, 0.0, x) for x in row] for row in xyz])
@@ -56,7 +60,7 @@ def test_fp32_parts_and_boundaries(solver, oracle, prec):
     m = mesh.export()
     for parts in (1, 2, 4, 7):
         Rm = solver.residual(parts=parts)
-        Ro = oracle.residual(m, 3, U, flux="roe", bc=bc, UD=UD, parts=parts)
+        Ro = oracle.residual(m, order, U, flux="roe", bc=bc, UD=UD, parts=parts)
         assert rel_err(Rm, Ro) < 2e-5, parts
 
 
@@ -122,3 +126,35 @@ def test_fp32_trajectory_tracks_fp64(solver, order, n, steps, prec):
         # relative to the largest conserved integral (the rho w integral of this flow is zero)
         drift = np.abs(np.asarray(c1) - np.asarray(c0)) / np.abs(c0).max()
         assert drift.max() < (1e-13 if p == "fp64" else 1e-11), (p, drift)
+
+
+@pytest.mark.parametrize("prec", MIXED)
+@pytest.mark.parametrize("order,n", [(2, 6), (3, 6), (4, 4)])
+def test_mixed_face_paths_agree(oracle, order, n, prec, monkeypatch):
+    """Every face path of the mixed stage (default, the LDS-staged face-flux pass, the LDS test
+    pass) within 2e-5 of the oracle residual and within 1e-6 of the default path's residual and
+    two-step state."""
+    print(f"[case] mixed face paths {prec} P{order}", flush=True)
+    mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=5)
+    U = mesh.project(order, "vortex", VORTEX)
+    U *= 1 + 1e-2 * np.random.default_rng(3).standard_normal(U.shape)
+    out = {}
+    for path in ("default", "lds", "lds_flux"):
+        monkeypatch.delenv("FVDG_FACE_TEST", raising=False)
+        monkeypatch.delenv("FVDG_FACE_FLUX", raising=False)
+        if path == "lds":
+            monkeypatch.setenv("FVDG_FACE_TEST", "lds")
+        elif path == "lds_flux":
+            monkeypatch.setenv("FVDG_FACE_FLUX", "lds")
+        s = F.Solver(0)
+        s.set_scheme(order, flux="roe", precision=prec).set_mesh(mesh)
+        s.upload(U)
+        R = s.residual()
+        s.ssprk3(0.01, 2)
+        out[path] = (R, s.download())
+        s.close()
+    Ro = oracle.residual(mesh.export(), order, U, flux="roe")
+    for path, (R, U2) in out.items():
+        assert rel_err(R, Ro) < 2e-5, path
+        assert rel_err(R, out["default"][0]) < 1e-6, path
+        assert rel_err(U2, out["default"][1]) < 1e-6, path
