@@ -1,24 +1,40 @@
 #!/usr/bin/env python3
 """Throughput of the explicit DG Euler RK stage on one MI355X.
 
-A step is one SSP-RK3 time step (3 stages; each stage = residual + inverse mass +
-stage update) over the whole mesh, with the state resident in HBM.  The metric is
-DOF-updates per second per RK stage: 5*N*E*3*K / t over K timed steps.
+A step is one SSP-RK3 time step (3 stages; each stage = residual + inverse mass + stage update)
+over the whole mesh, with the state resident in HBM.  The metric is DOF-updates per second per
+RK stage: 5*N*E*3*K / t over K timed steps (HIP events on the solver stream).
 
 Workloads (--workload):
-  vortex_p3_196k   (default) 32^3 hexes x 6 = 196,608-tet periodic cube [0,10]^3, P3, Roe
+  vortex_p3_1296k  (default) 60^3 x 6 = 1,296,000 tets, periodic [0,10]^3, P3, Roe, vertices
+                   jittered by +-10 % of h: the configuration the FP64-vs-mixed metric is quoted on
+  vortex_p3_196k   32^3 x 6 = 196,608 tets, P3, Roe
   vortex_p1_3k     8^3 x 6 = 3,072 tets, P1, Lax-Friedrichs
-  vortex_p3_1296k  60^3 x 6 = 1,296,000 tets, P3, Roe, vertices jittered by +-10 % of h
   tgv_p4_384k      40^3 x 6 = 384,000 tets on [0,2pi]^3, P4, Roe, Taylor-Green vortex (Mach 0.1)
   ensemble_256     256 independent vortex problems (n ~ U{6..24}, P1..P4, strength U(1,5),
-                   centre U(3,7)^2, seed 256), one HIP-graph replay per step, autotuned
-The vortex is the isentropic vortex (gamma 1.4, strength 5, centre (5,5), free stream
-(1,1,0), rho = p = 1 at infinity).  `value` is the FP64 path (the drop-in); the `mixed`
-object times the FP32-contraction path on the same workload and reports its relative L2
-deviation from the FP64 state after the same steps.
+                   centre U(3,7)^2, seed 256), HIP-graph replays, autotuned
+The vortex is the isentropic vortex (gamma 1.4, strength 5, centre (5,5), free stream (1,1,0),
+rho = p = 1 at infinity).  `value` is the FP64 path (the drop-in); `mixed` (FP32 contractions)
+and `mixed_bf16x3` time the mixed-precision paths on the same workload from the same initial
+state and report their relative L2 deviation from the FP64 state and, for the vortex, every
+path's relative L2 error against the exact (translated) vortex.
 
-Printed: ONE JSON line with the metric, the roofline of the dominant kernel (HIP events on
-the solver stream), and the CPU oracle timed on a bounded sample on this host's cores.
+Roofline (the algorithmic counts of the path, per element and stage, N modes, Q volume and Qf
+face points):
+  flops  F = 40QN + 145Q + 80QfN + 410Qf + 25N   (volume 40QN + 145Q; faces: traces 40QfN,
+         test 40QfN, Riemann 410Qf; update 25N)
+  bytes  B = 8 * 5N * (2 + 2/3) + 168            (state read and written, U^n read in two of
+         three stages, cell and face geometry)
+per kernel: the face-flux kernel F = 40QfN + 410Qf, B = 40N + 88 (its stage input and the face
+geometry); the element kernel (volume + face test + update) F = 40QN + 145Q + 40QfN + 25N,
+B = 40N (2 + 2/3) + 80.  `roofline` is the dominant (longest) kernel; `traffic` is its HBM bytes
+per launch measured by rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a device copy)
+for this workload, read from profiles/hbm_traffic_<workload>.txt (null when absent).  `stage`
+gives the same counts for the whole stage, with the measured bytes beside the algorithmic ones.
+
+Printed: ONE JSON line.  The CPU baseline (the oracle, a C restatement with the reference's loop
+structure, OpenMP) runs on this host: all threads on the full workload for at least one stage,
+and one thread on a 12^3 x 6 sample.
 """
 import argparse
 import json
@@ -49,50 +65,123 @@ WORKLOADS = {
 VORTEX = [1.4, 5.0, 5.0, 5.0, 1.0, 1.0, 10.0, 10.0]
 TGV = [1.4, 1.0, 1.0, 0.1]  # gamma, rho0, V0, Mach
 
-# HBM bytes per launch at the default workload (vortex_p3_196k, FP64) from rocprofv3 FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), profiles/dg_stage_p3_196k_fp64.md
-TRAFFIC = {
-    "vortex_p3_196k": {
-        "volume (k_vol_mfma)": (85029 * 2 + 153600) * 1024,
-        "face flux (k_face_flux_dpp)": (151599 * 2 + 376357) * 1024,
-        "face test + update (k_face_test_mfma)": (623421 * 2 + 176128) * 1024,
+
+def counts(Q, Qf, N):
+    """Algorithmic flops / bytes per element and stage: whole stage and per kernel slot."""
+    vol_f = 40 * Q * N + 145 * Q
+    flux_f = 40 * Qf * N + 410 * Qf
+    test_f = 40 * Qf * N
+    upd_f = 25 * N
+    st = 8 * 5 * N
+    return {
+        "stage": (vol_f + flux_f + test_f + upd_f, st * (2 + 2 / 3) + 168),
+        "volume": (vol_f, st + 80),
+        "flux": (flux_f, st + 88),
+        "test": (test_f + upd_f, st * (2 + 2 / 3) + 8),
+        "elem": (vol_f + test_f + upd_f, st * (2 + 2 / 3) + 80),
+        "contraction": 10 * Q * N + 30 * Q * N + 80 * Qf * N,  # flops of the dense contractions
     }
-}
 
 
-def test_kernel_name(precision, order):
-    """The test pass runs on the matrix cores up to P4 in FP64 and up to P3 in FP32 (unless
-    FVDG_FACE_TEST=lds), otherwise on the LDS kernel."""
-    mfma = order <= (4 if precision == "fp64" else 3) and os.environ.get("FVDG_FACE_TEST") != "lds"
-    return "k_face_test_mfma" if mfma else "k_face_test_lds"
+def read_traffic(workload):
+    """{kernel-name prefix: HBM bytes per launch} from profiles/hbm_traffic_<workload>.txt (lines
+    'name fetch_KiB write_KiB'; bytes = 1024 (2 fetch + write), the calibration of that file)."""
+    f = ROOT / "profiles" / f"hbm_traffic_{workload}.txt"
+    out = {}
+    if f.exists():
+        for line in f.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and not line.startswith("#"):
+                out[parts[0]] = 1024.0 * (2.0 * float(parts[1]) + float(parts[2]))
+    return out
 
 
-def flux_kernel_name(precision, order):
-    """The face-flux pass at P2-P4 holds the coefficients in registers and broadcasts them along
-    16-lane rows, with FP64 traces in every precision (unless FVDG_FACE_FLUX=lds); otherwise it
-    stages them in LDS."""
-    dpp = 2 <= order <= 4 and os.environ.get("FVDG_FACE_FLUX") != "lds"
-    return "k_face_flux_dpp" if dpp else "k_face_flux_lds"
+def traffic_of(table, kernel_name):
+    """measured bytes of a kernel (names compared without spaces), or None"""
+    key = kernel_name.replace(" ", "")
+    for k, v in table.items():
+        if k.replace(" ", "") == key:
+            return v
+    return None
 
 
-def algorithmic(Q, Qf, N, word=8):
-    """Algorithmic work per element and stage (FMA = 2 flops).
-    volume kernel      40QN + 145Q flops (interpolation 10QN, test 30QN, pointwise flux ~145Q);
-                       ~17 flop/B at P3: compute bound, counted against the matrix-core peak
-    face-flux kernel   40QfN + 410Qf flops per element (2 faces x [2 traces 10QfN + Riemann ~205Qf])
-    face-test kernel   reads U, R, 4 face-flux blocks, U^n (2 of 3 stages), |det|; writes U:
-                       8*(5N*(3 + 2/3) + 1) + word*4*Qf*5 bytes, ~3 flop/B: HBM bound."""
-    vol_flops = 40 * Q * N + 145 * Q
-    flux_flops = 40 * Qf * N + 410 * Qf
-    test_bytes = 8 * (5 * N * (3 + 2 / 3) + 1) + word * 4 * Qf * 5
-    return vol_flops, flux_flops, test_bytes
+def roofline(ms, names, E, Q, Qf, N, precision, traffic):
+    """Per-kernel and per-stage roofline of one precision's stage (ms from fvdg_time_stages)."""
+    c = counts(Q, Qf, N)
+    fused = not names.get("volume")
+    slots = [("volume", ms[1]), ("flux", ms[2]), ("elem" if fused else "test", ms[3])]
+    label = {"volume": "volume", "flux": "flux", "test": "test", "elem": "test"}
+    kernels = {}
+    for slot, t in slots:
+        if t <= 0:
+            continue
+        F, B = c[slot]
+        name = names.get(label[slot]) or slot
+        if precision == "fp64":
+            peak_f = PEAK_FP64
+        else:  # contractions at the FP32 matrix peak, the rest (pointwise, Riemann, update) at FP64
+            Fc = {"volume": 40 * Q * N, "flux": 40 * Qf * N, "test": 40 * Qf * N,
+                  "elem": 40 * Q * N + 40 * Qf * N}[slot]
+            peak_f = F / (Fc / PEAK_FP32 + (F - Fc) / PEAK_FP64)
+        t_s = t * 1e-3
+        tf, tb = F * E / peak_f, B * E / PEAK_HBM
+        bound = "mfma" if tf >= tb else "hbm"
+        measured = traffic_of(traffic, name) if traffic else None
+        kernels[name] = {
+            "ms": round(t, 4), "bound": bound,
+            "flops_per_elem": F, "bytes_per_elem": round(B, 1),
+            "achieved_tflops": round(F * E / t_s / 1e12, 3), "peak_tflops": round(peak_f / 1e12, 1),
+            "frac_flops": round(F * E / t_s / peak_f, 4),
+            "achieved_gbs": round(B * E / t_s / 1e9, 1), "frac_hbm": round(B * E / t_s / PEAK_HBM, 4),
+            "measured_bytes": measured,
+            "measured_gbs": round(measured / t_s / 1e9, 1) if measured else None,
+        }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    k = kernels[dom]
+    Fs, Bs = c["stage"]
+    t_st = ms[0] * 1e-3
+    measured_stage = sum(v["measured_bytes"] for v in kernels.values()) \
+        if all(v["measured_bytes"] for v in kernels.values()) else None
+    if precision == "fp64":
+        peak_stage = PEAK_FP64
+    else:
+        peak_stage = Fs / (c["contraction"] / PEAK_FP32 + (Fs - c["contraction"]) / PEAK_FP64)
+    mfma = k["bound"] == "mfma"
+    return {
+        "bound": k["bound"],
+        "achieved": k["achieved_tflops"] if mfma else k["achieved_gbs"],
+        "peak": k["peak_tflops"] if mfma else PEAK_HBM / 1e9,
+        "unit": "TFLOP/s" if mfma else "GB/s",
+        "frac": k["frac_flops"] if mfma else k["frac_hbm"],
+        "traffic": k["measured_bytes"],
+        "kernel": dom, "kernel_ms": k["ms"],
+        "counts": "algorithmic flops and bytes per element and stage (see bench.py docstring)",
+        "stage": {"ms": round(ms[0], 4), "flops_per_elem": Fs, "bytes_per_elem": round(Bs, 1),
+                  "achieved_tflops": round(Fs * E / t_st / 1e12, 3), "peak_tflops": round(peak_stage / 1e12, 1),
+                  "frac_flops": round(Fs * E / t_st / peak_stage, 4),
+                  "achieved_gbs": round(Bs * E / t_st / 1e9, 1), "frac_hbm": round(Bs * E / t_st / PEAK_HBM, 4),
+                  "measured_bytes": measured_stage,
+                  "measured_over_algorithmic_bytes": round(measured_stage / (Bs * E), 3) if measured_stage else None},
+        "kernels": kernels,
+    }
 
 
-def cpu_baseline(workload_n, order, flux, field, L, seconds_target=10.0):
-    """The oracle (C restatement, OpenMP, reference loop structure) on a bounded
-    sample of the same workload: a periodic box of min(n, 16)^3 x 6 tets at the same
-    order and flux; RK stages (residual + inverse mass + update) repeated until
-    `seconds_target` of CPU work, median stage time."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mesh, U0, order, flux, field, L, seconds_target=20.0):
+    """The oracle (C restatement of ExplicitConvection::eval with the reference's loop structure:
+    per-point basis evaluation, a 3x3 inverse per (point, test function), OpenMP dynamic schedule
+    with atomic scatter) timed on this host: RK stages (residual + inverse mass + update) of the
+    full workload with all threads until `seconds_target` (at least one), and of a 12^3 x 6 sample
+    with one thread (three stages).  Median stage time each."""
     import numpy as np
 
     import fve_dg_amd as F
@@ -100,55 +189,35 @@ def cpu_baseline(workload_n, order, flux, field, L, seconds_target=10.0):
 
     O = Oracle()
     threads = O.max_threads()
-    n = min(workload_n, 16)
-    mesh = F.Mesh.box(n, ub=(L, L, L), periodic=7)
+
+    def stages(m, U, th, n_min, budget):
+        M = O.mass(m, order)
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < n_min or (time.perf_counter() - t_start < budget and len(times) < 8):
+            t0 = time.perf_counter()
+            R = O.residual(m, order, U, flux=flux, threads=th)
+            U = U + 1e-4 * (-R / M)
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), len(times), sum(times)
+
+    N = F.num_basis(order)
     m = mesh.export()
-    U = mesh.project(order, field, VORTEX if field == "vortex" else TGV)
-    M = O.mass(m, order)
     E = len(m["cells"])
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 or time.perf_counter() - t_start < seconds_target:
-        t0 = time.perf_counter()
-        R = O.residual(m, order, U, flux=flux, threads=threads)
-        U = U + 1e-4 * (-R / M)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    dof = 5 * F.num_basis(order) * E
-    return {"value": dof / t, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} RK stages on a {n}^3x6 = {E}-tet periodic box, P{order} {flux}, "
-                      f"oracle/dg_oracle.c with {threads} OpenMP threads; median stage {t:.3f} s of "
-                      f"{sum(times):.1f} s total"}
-
-
-def kernel_roofline(workload, ms, E, Q, Qf, N, precision, order):
-    vol_f, flux_f, test_b = algorithmic(Q, Qf, N, 8 if precision == "fp64" else 4)
-    peak_mm = PEAK_FP64 if precision == "fp64" else PEAK_FP32
-
-    def rate(work, t_ms):
-        return work * E / (t_ms * 1e-3) if t_ms > 0 else 0.0
-
-    kernels = {
-        "volume (k_vol_mfma)": {"ms": ms[1], "achieved": rate(vol_f, ms[1]), "peak": peak_mm, "bound": "mfma",
-                                "unit": "TFLOP/s", "scale": 1e12},
-        f"face flux ({flux_kernel_name(precision, order)})": {
-            "ms": ms[2], "achieved": rate(flux_f, ms[2]), "peak": PEAK_FP64, "bound": "mfma", "unit": "TFLOP/s",
-            "scale": 1e12},
-        f"face test + update ({test_kernel_name(precision, order)})": {
-            "ms": ms[3], "achieved": rate(test_b, ms[3]), "peak": PEAK_HBM, "bound": "hbm", "unit": "GB/s",
-            "scale": 1e9},
-    }
-    dom = max(kernels, key=lambda k: kernels[k]["ms"])
-    k = kernels[dom]
-    traffic = TRAFFIC.get(workload, {}).get(dom) if precision == "fp64" else None
-    return {"bound": k["bound"], "achieved": round(k["achieved"] / k["scale"], 3),
-            "peak": round(k["peak"] / k["scale"], 1), "unit": k["unit"],
-            "frac": round(k["achieved"] / k["peak"], 4), "traffic": traffic,
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
-            "kernel": dom, "kernel_ms": round(k["ms"], 4),
-            "all_kernels": {name: {"ms": round(v["ms"], 4), "achieved": round(v["achieved"] / v["scale"], 3),
-                                   "unit": v["unit"], "frac": round(v["achieved"] / v["peak"], 4)}
-                            for name, v in kernels.items()}}
+    t_all, n_all, s_all = stages(m, U0, threads, 1, seconds_target)
+    small = F.Mesh.box(12, ub=(L, L, L), periodic=7)
+    ms_ = small.export()
+    Us = small.project(order, field, VORTEX if field == "vortex" else TGV)
+    t_one, n_one, s_one = stages(ms_, Us, 1, 3, 0.0)
+    Es = len(ms_["cells"])
+    return {"value": 5 * N * E / t_all, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"{n_all} RK stage(s) of the full {E}-tet workload, P{order} {flux}, oracle/dg_oracle.c "
+                      f"(-O3 -mavx2 -mfma -fopenmp) with {threads} OpenMP threads; median stage {t_all:.2f} s of "
+                      f"{s_all:.1f} s",
+            "single_thread": {"value": 5 * N * Es / t_one, "unit": "DOF-updates/s", "cores": 1,
+                              "sample": f"{n_one} RK stages of a 12^3x6 = {Es}-tet periodic box, P{order} {flux}; "
+                                        f"median stage {t_one:.2f} s"}}
 
 
 def run_single(args, w):
@@ -172,6 +241,8 @@ def run_single(args, w):
     print(f"[bench] {args.workload}: E={E} P{order} {flux} dt={dt:.3e} setup {time.perf_counter() - t_setup:.1f}s",
           file=sys.stderr, flush=True)
     dof = 5 * N * E
+    traffic = read_traffic(args.workload)
+    steps_done = args.warmup + 2 * args.steps  # fvdg_time_stages runs two passes of `steps` steps
     res = {}
     for prec in ("fp64", "fp32", "bf16x3"):
         s.set_scheme(order, flux=flux, precision=prec)
@@ -185,9 +256,12 @@ def run_single(args, w):
         if not np.isfinite(U).all():
             print(f"[bench] non-finite state after the timed steps ({prec})", file=sys.stderr)
             sys.exit(3)
-        res[prec] = dict(ms=ms, wall=wall, U=U, value=dof / (ms[0] * 1e-3))
-        print(f"[bench] {prec}: {ms[0]:.4f} ms/stage, {res[prec]['value']:.4g} DOF-updates/s", file=sys.stderr,
-              flush=True)
+        r = dict(ms=ms, wall=wall, U=U, value=dof / (ms[0] * 1e-3), names=s.stage_kernels())
+        if w["field"] == "vortex":
+            r["analytic"] = mesh.analytic_error(order, "vortex", params, steps_done * dt, U)
+        res[prec] = r
+        print(f"[bench] {prec}: {ms[0]:.4f} ms/stage ({ms[1]:.4f} / {ms[2]:.4f} / {ms[3]:.4f}), "
+              f"{r['value']:.4g} DOF-updates/s, kernels {r['names']}", file=sys.stderr, flush=True)
     # the same FP64 stages with the limiter (apply_1 + apply_2, point bounds) after each stage
     s.set_scheme(order, flux=flux, precision="fp64")
     s.upload(U0)
@@ -195,12 +269,23 @@ def run_single(args, w):
     ms_lim = s.time_stages(dt, 3 * args.steps)
     s.set_stage_limiter(0)
     s.close()
-    f64, f32, b3 = res["fp64"], res["fp32"], res["bf16x3"]
+    f64 = res["fp64"]
 
     def deviation(r):
         return float(np.sqrt(((r["U"] - f64["U"]) ** 2).sum() / (f64["U"] ** 2).sum()))
 
-    dev = deviation(f32)
+    def analytic(r):
+        # relative L2 error per variable (rho, rho u, rho v, rho E; the exact rho w is zero)
+        return None if "analytic" not in r else [float(r["analytic"][i]) for i in (0, 1, 2, 4)]
+
+    def mixed_block(prec, label):
+        r = res[prec]
+        return {"precision": label, "value": round(r["value"], 1), "ms_per_step": round(3 * r["ms"][0], 4),
+                "speedup_vs_fp64": round(r["value"] / f64["value"], 4),
+                "l2_rel_deviation_vs_fp64": deviation(r), "analytic_l2_rel": analytic(r),
+                "after_steps": steps_done, "kernels": r["names"],
+                "roofline": roofline(r["ms"], r["names"], E, Q, Qf, N, prec, traffic)}
+
     out = {
         "metric": METRIC,
         "value": round(f64["value"], 1),
@@ -216,25 +301,20 @@ def run_single(args, w):
         "data": f"synthetic ({'isentropic vortex' if w['field'] == 'vortex' else 'Taylor-Green vortex'} projected "
                 f"on the mesh)",
         "config": {"workload": args.workload, "tets": E, "order": order, "modes": N, "flux": flux, "dofs": dof,
-                   "stages_per_step": 3, "dt": dt, "cfl": 0.3, "wall_s": round(f64["wall"], 3)},
-        "roofline": kernel_roofline(args.workload, f64["ms"], E, Q, Qf, N, "fp64", order),
-        "mixed": {"precision": "fp32 contractions, fp64 fluxes/update/state",
-                  "value": round(f32["value"], 1), "ms_per_step": round(3 * f32["ms"][0], 4),
-                  "speedup_vs_fp64": round(f32["value"] / f64["value"], 4),
-                  "l2_rel_deviation_vs_fp64": dev, "after_steps": args.warmup + args.steps,
-                  "roofline": kernel_roofline(args.workload, f32["ms"], E, Q, Qf, N, "fp32", order)},
-        "mixed_bf16x3": {"precision": "volume contractions as 6 bf16 MFMA products of 3-part splits (fp32 "
-                                      "accumulation), face contractions fp32, fp64 fluxes/update/state",
-                         "value": round(b3["value"], 1), "ms_per_step": round(3 * b3["ms"][0], 4),
-                         "speedup_vs_fp64": round(b3["value"] / f64["value"], 4),
-                         "l2_rel_deviation_vs_fp64": deviation(b3), "after_steps": args.warmup + args.steps,
-                         "volume_kernel_ms": round(b3["ms"][1], 4)},
+                   "stages_per_step": 3, "dt": dt, "cfl": 0.3, "jitter": w["jitter"],
+                   "wall_s": round(f64["wall"], 3), "kernels": f64["names"],
+                   "analytic_l2_rel": analytic(f64), "after_steps": steps_done},
+        "roofline": roofline(f64["ms"], f64["names"], E, Q, Qf, N, "fp64", traffic),
+        "mixed": mixed_block("fp32", "FP32 matrix-core contractions (FP32 accumulation), FP32 face traces, FP64 "
+                                     "state / pointwise flux / Riemann central flux and pressure / update / sums"),
+        "mixed_bf16x3": mixed_block("bf16x3", "volume contractions as 6 bf16 matrix-core products of 3-part splits "
+                                              "(FP32 accumulation), face contractions FP32, FP64 state and fluxes"),
         "limited": {"limiter": "PositiveLimiter apply_1 + apply_2 after every stage (fp64)",
                     "ms_per_step": round(3 * ms_lim[0], 4),
                     "overhead_vs_fp64": round(ms_lim[0] / f64["ms"][0] - 1, 4)},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n, order, flux, w["field"], L)
+        out["cpu_baseline"] = cpu_baseline(mesh, U0, order, flux, w["field"], L)
     return out
 
 
@@ -312,9 +392,9 @@ def run_ensemble(args, w):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="vortex_p3_196k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="vortex_p3_1296k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before the run, copy MIB MiB device-to-device once (a plain copy kernel of known "

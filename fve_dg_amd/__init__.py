@@ -94,6 +94,7 @@ def lib():
         _sig(L, "fvdg_max_wave_speed", [_vp, _d])
         _sig(L, "fvdg_conservation_sums", [_vp, _d])
         _sig(L, "fvdg_time_stages", [_vp, C.c_double, C.c_int, _d])
+        _sig(L, "fvdg_stage_kernels", [_vp, C.c_char_p, C.c_int])
         _sig(L, "fvdg_flow_diagnostics", [_vp, _d])
         _ip = C.POINTER(C.c_int)
         _sig(L, "fvdg_implicit_assemble", [_vp, C.c_int])
@@ -374,6 +375,12 @@ class Solver:
         out = np.zeros(5)
         _check(lib().fvdg_conservation_sums(self._h, _p(out)))
         return out
+
+    def stage_kernels(self):
+        """dict slot -> kernel name ('volume', 'flux', 'test'; '' = nothing launched) of one stage"""
+        buf = C.create_string_buffer(512)
+        _check(lib().fvdg_stage_kernels(self._h, buf, 512))
+        return dict(item.split("=", 1) for item in buf.value.decode().split(";"))
 
     def time_stages(self, dt, n_stages):
         ms = np.zeros(4)

@@ -96,6 +96,8 @@ struct fvdg_ctx {
     bool owner_faces = false;        // FVDG_FACES=owner: one-pass owner-computes face kernel
     bool lds_face_test = false;      // FVDG_FACE_TEST=lds: the LDS/VALU face test kernel
     bool lds_face_flux = false;      // FVDG_FACE_FLUX=lds: the LDS-staged face-flux kernel at P2-P4 (FP64)
+    bool elem_fused = true;          // k_elem_mfma after the face-flux pass (FVDG_ELEM=split: volume kernel +
+                                     // face-test kernel)
     DevBuf<double> red;              // reduction partials
     DevBuf<int32_t> bad;             // 1 word: set by the stage update when an output is non-finite or a
                                      // mean density <= 0 (cleared and read by fvdg_ssprk3_step/_stage)

@@ -1,0 +1,315 @@
+// k_elem_mfma<ORDER,T,FUSE>: one kernel per element tile for everything after the face-flux pass:
+// the volume term, the face test functions and (FUSE) the SSP-RK stage update.
+//
+// One wavefront owns 16 consecutive elements.  All contractions accumulate into ONE set of
+// matrix-core accumulators R_v[j][e] (rows = test functions j, columns = the tile's elements):
+//   volume interpolation  U_v[q][e] = sum_k V[q][k] C_v[k][e]                 (16x16x4 products)
+//   volume test           R_v[j][e] += sum_(q,d) G[q][d][j] B_v[(q,d)][e]
+//                         B_v[(q,d)][e] = -w_q |J_e| (F(U_e(xi_q)) A_e^{-1})_(v,d), pointwise, FP64
+//   face test             R_v[j][e] += sum_slots sum_g' PhiC_c[g'][j] Fh_slot[pinv_m(g')][v]
+// The face test has the canonical face table as the A operand (per-lane layout PhiB[c][ks][jt][lane]:
+// table point g' = 4 ks + lane/16, test function j = 16 jt + lane%16) and the face fluxes of the
+// tile's elements as the B operand (column = element, K = canonical point g'), so its products land
+// in the volume term's accumulator layout.  Each element slot sees its face through one of the eight
+// canonical tables c and a point rotation m; per slot the product runs once per table in use by the
+// tile (wave-uniform ballot) with the B columns of elements on another table zero.
+// The volume residual therefore never goes to memory, and the stage input U is read once: its
+// tile stays in LDS (FP64) for the interpolation and the update.  Each lane then turns its
+// accumulators (test function j, element e) into the stage value U + dt (-R / M), M = |det A_e|
+// mhat[j] (FUSE; the residual itself otherwise), in place in the LDS tile, after adding the FP64
+// parts of the mixed path; a last pass over the tile in contiguous runs writes
+// out = a0 U0 + b0 (stage value) (or R) with the stage-output check (non-finite value, mean
+// density <= 0).
+//
+// T = double: FP64 matrix cores (the drop-in).  T = float (mixed): FP32 matrix cores with the
+// splits of the mixed path kept in FP64 -- the interpolation adds the cell mean (mode 0) in FP64,
+// the volume test runs on F(U_q) - F(c_0) with the mean flux's term -|J| (F(c_0) A^{-1}) .
+// sum_q w_q grad phi_j added in FP64, and each face's FP64 reference flux enters as
+// sum_slots (+-Fref) sum_g w_g PhiC_c[g][j] (table IC).
+#pragma once
+#include "fvdg_kernels_fast.hpp"
+
+template <typename T>
+struct ElemArgs {
+    int64_t E;
+    double gamma;
+    int parts;
+    const double* U;      // stage input (FP64)
+    double* R;            // residual output (FUSE = false)
+    const double* cgeo;   // E*10: A^{-1}, |det A|
+    const int32_t* cfaces;
+    const uint8_t* cmeta;
+    const T* VA;          // volume tables, zero padded MFMA operands
+    const T* GA;
+    const double* wq;
+    const double* Gs;     // N*3 (mixed)
+    const T* Fh;          // face fluxes [F][Qf][5] (mixed: differences from Fref)
+    const double* Fref;   // [F][5] (mixed)
+    const double* IC;     // [8][N] (mixed)
+    const T* PhiB;        // canonical face tables as per-lane MFMA A operands
+    const uint8_t* canon;
+    const int32_t* perm;
+    // fused SSP-RK stage
+    const double* U0;
+    double* out;
+    const double* mhat;
+    double a0, b0, dt;
+    const double* edt;
+    int* bad;
+};
+
+template <int ORDER, typename T>
+struct ElemShape {
+    static constexpr int N = Ord<ORDER>::N;
+    // two waves per SIMD: the tile's LDS (16 x 5N doubles, 12.9 KB at P3) allows about that many
+    static constexpr int MIN_WAVES = 2;
+    static constexpr bool SUPPORTED = N <= 35;
+};
+
+template <int ORDER, typename T, bool FUSE>
+__global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_mfma(ElemArgs<T> a) {
+    using S = VolShape<ORDER>;
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
+    constexpr int N = S::N, Q = S::Q, NV = S::NV, QT = S::QT, KS = S::KS, JT = S::JT;
+    constexpr int KPAD = S::KPAD, JPAD = S::JPAD, STR = S::STRIDE;
+    constexpr int QF = Ord<ORDER>::QF, FKS = (QF + 3) / 4;
+    __shared__ double sU[16 * STR];  // the tile's coefficients (FP64), later its stage values (or residual)
+    __shared__ double sRef[SPLIT ? 16 * 4 * 5 : 1];
+    __shared__ int sSlotC[SPLIT ? 16 * 4 : 1];
+    __shared__ int sPinv[3 * QF];
+    __shared__ int sCanon[24];
+    const int l = threadIdx.x, col = l & 15, grp = l >> 4;
+    for (int i = l; i < 3 * QF; i += 64) {
+        const int m = i / QF, g = i - m * QF;
+        sPinv[m * QF + a.perm[i]] = g;
+    }
+    if (l < 24) sCanon[l] = a.canon[l];
+    const int64_t e0 = (int64_t)blockIdx.x * 16;
+    const int64_t base = e0 * NV, limit = a.E * NV;
+    const bool do_cells = (a.parts & FVDG_PART_CELLS) != 0;
+    const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
+
+    // 1. the tile's coefficients (16 NV contiguous values) in LDS
+    stage_to_lds<16 * NV, 64>(
+        l, [&](int i) { return (base + i < limit) ? a.U[base + i] : 0.0; },
+        [&](int i, double x) { sU[(i / NV) * STR + (i % NV)] = x; });
+    __syncthreads();
+
+    // 2. geometry of this lane's element (column col); split: mean state and its flux
+    const int64_t e = e0 + col;
+    double Ai[9], det = 0.0, c0[5] = {0, 0, 0, 0, 0}, FrA[5][3];
+    if (e < a.E) {
+        const double* g = a.cgeo + e * 10;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ai[i] = g[i];
+        det = g[9];
+        if (SPLIT)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) c0[v] = a.U[e * NV + v];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ai[i] = 0.0;
+    }
+    if (SPLIT) {
+        double F[5][3];
+        euler_flux(c0, a.gamma, F);
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) FrA[v][d] = F[v][0] * Ai[d] + F[v][1] * Ai[3 + d] + F[v][2] * Ai[6 + d];
+    }
+
+    acc_t racc[5][JT];
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) racc[v][jt] = (acc_t){0, 0, 0, 0};
+
+    // 3. volume term
+    if (do_cells) {
+#pragma unroll 1
+        for (int t = 0; t < QT; ++t) {
+            acc_t uacc[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) uacc[v] = (acc_t){0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const T av = a.VA[(16 * t + col) * KPAD + 4 * s + grp];
+                const int k = 4 * s + grp;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    // split: mode 0 (the cell mean) is added in FP64 at the points
+                    const T bv = (k < N && (!SPLIT || k > 0)) ? (T)sU[col * STR + 5 * k + v] : (T)0;
+                    uacc[v] = M::mma(av, bv, uacc[v]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = 16 * t + M::row(r, grp);
+                T B[5][3];
+                if (q < Q) {
+                    double Uq[5];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) Uq[v] = c0[v] + (double)uacc[v][r];
+                    double F[5][3];
+                    euler_flux(Uq, a.gamma, F);
+                    const double jw = -a.wq[q] * det;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            double fa = F[v][0] * Ai[d] + F[v][1] * Ai[3 + d] + F[v][2] * Ai[6 + d];
+                            if (SPLIT) fa -= FrA[v][d];
+                            B[v][d] = (T)(fa * jw);
+                        }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) B[v][d] = (T)0;
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                    for (int jt = 0; jt < JT; ++jt) {
+                        const T av = a.GA[(q * 3 + d) * JPAD + 16 * jt + col];
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) racc[v][jt] = M::mma(av, B[v][d], racc[v][jt]);
+                    }
+                }
+            }
+        }
+    }
+
+    // 4. face test: the four slots of this lane's element (column col)
+    int4 cf4 = make_int4(-1, -1, -1, -1);
+    uchar4 cm4 = make_uchar4(0, 0, 0, 0);
+    if (e < a.E) {
+        cf4 = *reinterpret_cast<const int4*>(a.cfaces + e * 4);
+        cm4 = *reinterpret_cast<const uchar4*>(a.cmeta + e * 4);
+    }
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        int cl = -1, m = 0;
+        int64_t f = 0;
+        T sign = 1;
+        if (e < a.E) {
+            const int32_t slot = s == 0 ? cf4.x : s == 1 ? cf4.y : s == 2 ? cf4.z : cf4.w;
+            const int meta = s == 0 ? cm4.x : s == 1 ? cm4.y : s == 2 ? cm4.z : cm4.w;
+            const bool bnd = (meta & 0x80) != 0;
+            if ((bnd && do_bnd) || (!bnd && do_int)) {
+                const int cb = sCanon[meta & 0x1f];
+                cl = cb / 3;
+                m = cb - 3 * cl;
+                f = slot >> 1;
+                sign = (slot & 1) ? -1 : 1;
+            }
+        }
+        if (SPLIT && grp == 0) {
+            sSlotC[col * 4 + s] = cl;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sRef[(col * 4 + s) * 5 + v] = cl >= 0 ? (double)sign * a.Fref[f * 5 + v] : 0.0;
+        }
+        unsigned used = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) used |= (__ballot(cl == c) != 0 ? 1u : 0u) << c;
+        // per table in use: the lanes of elements on that table load their face fluxes (every lane
+        // exactly once per slot), all lanes the table's operands; loads of all K steps are issued
+        // before the products
+        for (unsigned u = used; u != 0; u &= u - 1) {
+            const int c = __builtin_ctz(u);
+            const bool mine = cl == c;
+            T x[FKS][5], b[FKS][JT];
+#pragma unroll
+            for (int ks = 0; ks < FKS; ++ks) {
+                const int gc = 4 * ks + grp;
+                if (mine && gc < QF) {
+                    const T* p = a.Fh + (f * QF + sPinv[m * QF + gc]) * 5;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) x[ks][v] = sign * p[v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) x[ks][v] = 0;
+                }
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt) b[ks][jt] = a.PhiB[((c * FKS + ks) * JT + jt) * 64 + l];
+            }
+#pragma unroll
+            for (int ks = 0; ks < FKS; ++ks)
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+#pragma unroll
+                    for (int jt = 0; jt < JT; ++jt) racc[v][jt] = M::mma(b[ks][jt], x[ks][v], racc[v][jt]);
+        }
+    }
+
+    // 5. lane (j = 16 jt + row(r, grp), element col): R_v[j] from the accumulators, plus (split) the
+    //    mean flux's volume term and the faces' reference fluxes in FP64; FUSE: the stage value
+    //    U + dt (-R / M) replaces U in the LDS tile (the same lane reads and writes the entry)
+    __syncthreads();  // the slots' reference fluxes are in LDS
+    double fref[4][5];
+    int fc[4] = {-1, -1, -1, -1};
+    if (SPLIT) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            fc[s] = sSlotC[col * 4 + s];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) fref[s][v] = sRef[(col * 4 + s) * 5 + v];
+        }
+    }
+    const double dte = FUSE ? (a.edt ? (e < a.E ? a.edt[e] : 0.0) : a.dt) : 0.0;
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = 16 * jt + M::row(r, grp);
+            if (j < N) {
+                double rr[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) rr[v] = (double)racc[v][jt][r];
+                if (SPLIT) {
+                    if (do_cells) {
+                        const double g0 = a.Gs[3 * j], g1 = a.Gs[3 * j + 1], g2 = a.Gs[3 * j + 2];
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) rr[v] -= det * (FrA[v][0] * g0 + FrA[v][1] * g1 + FrA[v][2] * g2);
+                    }
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        if (fc[s] >= 0) {
+                            const double ic = a.IC[fc[s] * N + j];
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) rr[v] += ic * fref[s][v];
+                        }
+                }
+                double* p = sU + col * STR + 5 * j;
+                if (FUSE) {
+                    const double inv_m = e < a.E ? 1.0 / (det * a.mhat[j]) : 0.0;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) p[v] = p[v] + dte * (-rr[v] * inv_m);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) p[v] = rr[v];
+                }
+            }
+        }
+    __syncthreads();
+
+    // 6. out = a0 U0 + b0 (stage value), or R, over the tile in contiguous runs
+    bool badl = false;
+    const int64_t nvalid = max<int64_t>(0, min<int64_t>(16, a.E - e0)) * NV;
+    for (int i = l; i < nvalid; i += 64) {
+        const int ee = i / NV, kv = i - ee * NV;
+        const int64_t gi = base + i;
+        const double x = sU[ee * STR + kv];
+        if (FUSE) {
+            const double o = (a.a0 == 0.0) ? a.b0 * x : a.a0 * a.U0[gi] + a.b0 * x;
+            a.out[gi] = o;
+            badl |= stage_value_bad(o, kv);
+        } else {
+            a.R[gi] = x;
+        }
+    }
+    if (FUSE && badl && a.bad) a.bad[0] = 1;
+}

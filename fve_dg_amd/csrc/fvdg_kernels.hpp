@@ -199,6 +199,89 @@ __device__ __forceinline__ void num_flux(int kind, const double UL[5], const dou
         flux_lf(UL, UR, n, gamma, out);
 }
 
+// ------------------------------------------------------------------ mixed-precision numerical fluxes
+// The mixed path keeps FP64 where the flux value itself is formed -- the primitives (1/rho), the
+// pressures and positivity tests, the physical normal fluxes F(U_L) n and F(U_R) n and the jumps
+// U_R - U_L -- and evaluates the upwind dissipation, which is proportional to the jumps, in FP32
+// (Roe averages, sound speed, eigenvalues, wave strengths; hardware square root and reciprocal).
+// Its FP32 rounding is relative to the jump, not to the flux.
+__device__ __forceinline__ float sqrt_f32(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ void flux_lf_mx(const double UL[5], const double UR[5], const double n[3], double gamma,
+                                           double out[5]) {
+    const Prim L = euler_prim(UL, gamma), R = euler_prim(UR, gamma);
+    double fl[5], fr[5];
+    euler_flux_n(L, n, fl);
+    euler_flux_n(R, n, fr);
+    const float g = (float)gamma;
+    auto speed = [g](const Prim& q) {
+        const float u = (float)q.u, v = (float)q.v, w = (float)q.w;
+        const float c = sqrt_f32(g * fmaxf((float)(q.p * q.ri), 1e-16f));
+        return sqrt_f32(u * u + v * v + w * w) + c;
+    };
+    const double lam = (double)fmaxf(speed(L), speed(R));
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fl[v] + fr[v] + lam * (UL[v] - UR[v]));
+}
+
+__device__ __forceinline__ void flux_roe_mx(const double UL[5], const double UR[5], const double n[3], double gamma,
+                                            double out[5]) {
+    const Prim L = euler_prim(UL, gamma), R = euler_prim(UR, gamma);
+    if (!(UL[0] > 0.0 && UR[0] > 0.0 && L.p > 0.0 && R.p > 0.0)) {
+        flux_lf_mx(UL, UR, n, gamma, out);
+        return;
+    }
+    double fl[5], fr[5];
+    euler_flux_n(L, n, fl);
+    euler_flux_n(R, n, fr);
+    const float g = (float)gamma;
+    const float nf[3] = {(float)n[0], (float)n[1], (float)n[2]};
+    const float uL[3] = {(float)L.u, (float)L.v, (float)L.w}, uR[3] = {(float)R.u, (float)R.v, (float)R.w};
+    const float HL = (float)((UL[4] + L.p) * L.ri), HR = (float)((UR[4] + R.p) * R.ri);
+    const float sL = sqrt_f32((float)L.rho), sR = sqrt_f32((float)R.rho), inv = rcp_f32(sL + sR);
+    float ut[3], q2 = 0, qn = 0;
+    for (int d = 0; d < 3; ++d) {
+        ut[d] = (sL * uL[d] + sR * uR[d]) * inv;
+        q2 += ut[d] * ut[d];
+        qn += ut[d] * nf[d];
+    }
+    const float Ht = (sL * HL + sR * HR) * inv;
+    const float a2 = fmaxf((g - 1) * (Ht - 0.5f * q2), 1e-16f);
+    const float a = sqrt_f32(a2), ia2 = rcp_f32(a2);
+    const float rt = sL * sR;
+    const float dr = (float)(R.rho - L.rho), dp = (float)(R.p - L.p);
+    const float du[3] = {(float)(R.u - L.u), (float)(R.v - L.v), (float)(R.w - L.w)};
+    const float dqn = du[0] * nf[0] + du[1] * nf[1] + du[2] * nf[2];
+    float dut[3], ut_dut = 0;
+    for (int d = 0; d < 3; ++d) {
+        dut[d] = du[d] - dqn * nf[d];
+        ut_dut += ut[d] * dut[d];
+    }
+    const float a1 = (dp - rt * a * dqn) * (0.5f * ia2);
+    const float a5 = (dp + rt * a * dqn) * (0.5f * ia2);
+    const float a_2 = dr - dp * ia2;
+    const float delta = 0.1f * a;
+    auto fix = [delta](float lam) {
+        const float al = fabsf(lam);
+        return al < delta ? (lam * lam + delta * delta) * rcp_f32(2 * delta) : al;
+    };
+    const float l1 = fix(qn - a), l2 = fabsf(qn), l5 = fix(qn + a);
+    const float w1 = l1 * a1, w2 = l2 * a_2, w5 = l5 * a5, ws = l2 * rt;
+    float d[5];
+    d[0] = w1 + w2 + w5;
+    for (int k = 0; k < 3; ++k) d[1 + k] = w1 * (ut[k] - a * nf[k]) + w2 * ut[k] + ws * dut[k] + w5 * (ut[k] + a * nf[k]);
+    d[4] = w1 * (Ht - a * qn) + w2 * 0.5f * q2 + ws * ut_dut + w5 * (Ht + a * qn);
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * (double)d[v];
+}
+
+__device__ __forceinline__ void num_flux_mx(int kind, const double UL[5], const double UR[5], const double n[3],
+                                            double gamma, double out[5]) {
+    if (kind == FVDG_FLUX_ROE)
+        flux_roe_mx(UL, UR, n, gamma, out);
+    else
+        flux_lf_mx(UL, UR, n, gamma, out);
+}
+
 // ------------------------------------------------------------------ volume term
 // one thread per (element, volume point): trace of U and its physical flux
 #ifndef FVDG_DEVICE_HELPERS_ONLY

@@ -568,7 +568,6 @@ struct FaceFluxArgs {
     T* Fh;         // F*QF*5
     double* Fref;  // F*5 (FP32 path only)
     int64_t ntiles;
-    const double* PhiC64;  // k_face_flux_dpp: the canonical tables in FP64 (its traces are FP64)
 };
 
 template <int ORDER, typename T = double>
@@ -754,15 +753,19 @@ __global__ __launch_bounds__((FaceFluxShape<ORDER, T>::THREADS), (FaceFluxShape<
 //   k_face_flux_lds; with T = float (the mixed-precision stage) the traces and fluxes stay FP64
 //   and the face's reference flux (its point 0, held by lane 0 of the row) reaches the row by one
 //   more broadcast: Fref = F_0 2A in FP64, Fh[g] = (F_g - F_0) w_g 2A in FP32.
-template <int ORDER>
+template <int ORDER, typename T = double>
 struct FaceFluxDppShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
     static constexpr int GP = (QF + 15) / 16, LPF = (QF + GP - 1) / GP;  // points per lane, lanes per face
     static constexpr int CR = (NV + 15) / 16;                           // coefficient registers per side
     static constexpr int THREADS = 256, FB = THREADS / 16;              // faces per block and tile
-    // registers held to 3 waves per SIMD up to P3 (44 bytes per lane spilled, measured faster than
-    // 2 waves without spills); P4's tables (56 KB) allow 2 blocks per CU anyway
-    static constexpr int MIN_WAVES = N <= 20 ? 3 : 2;
+    // FP64: registers held to 3 waves per SIMD up to P3 (measured faster than 2 waves; P4's tables,
+    // 56 KB, allow 2 blocks per CU anyway), each tile's coefficients loaded at its start.  FP32 traces
+    // (mixed): the next tile's face and coefficients are loaded during the current tile's Riemann
+    // solves, with 2 waves per SIMD (P3 196k: 0.290 ms vs 0.322 at 3 waves with spills and 0.378
+    // without the prefetch; the FP64 kernel measured 0.312 without and 0.321 with it)
+    static constexpr bool PREFETCH = !std::is_same<T, double>::value;
+    static constexpr int MIN_WAVES = (N <= 20 && !PREFETCH) ? 3 : 2;
     static constexpr bool SUPPORTED = QF <= 32;
 };
 
@@ -785,30 +788,90 @@ __device__ __forceinline__ double row_bcast(double x) {
     return __builtin_bit_cast(double, ((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// One mode k of a trace sum for two face points of one side:
+//   acc[i][v] += c_v(broadcast from lane L_v = (5k + v) % 16 of the row) * t[i],  i = 0, 1, v = 0..4,
+// as ten v_fmac_f64_dpp / v_fmac_f32_dpp row_newbcast instructions: the broadcast is the DPP source
+// operand of the FMA itself (the compiler does not fuse a DPP move into a multiply-add, and
+// spends two 32-bit moves per double on the broadcast otherwise).  DPP reads of a VGPR need two
+// wait states after a VALU write of it: the block opens with s_nop 1, and no instruction of the
+// block reads a register another one of it writes.  Ten accumulators, five coefficient registers
+// (c_v lives in register (5k + v) / 16 of the side's coefficient set) and two table values.
+// The lane pattern repeats with period 16 in k: dpp_mode<k % 16>.
+#define FVDG_DPP_I(OP, ACC, C, T, L) OP " %" #ACC ", %" #C ", %" #T " row_newbcast:" #L " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+#define FVDG_DPP_BODY(OP, L0, L1, L2, L3, L4)                                                       \
+    "s_nop 1\n" FVDG_DPP_I(OP, 0, 10, 15, L0) FVDG_DPP_I(OP, 5, 10, 16, L0) FVDG_DPP_I(OP, 1, 11, 15, L1) \
+        FVDG_DPP_I(OP, 6, 11, 16, L1) FVDG_DPP_I(OP, 2, 12, 15, L2) FVDG_DPP_I(OP, 7, 12, 16, L2)       \
+            FVDG_DPP_I(OP, 3, 13, 15, L3) FVDG_DPP_I(OP, 8, 13, 16, L3) FVDG_DPP_I(OP, 4, 14, 15, L4)   \
+                FVDG_DPP_I(OP, 9, 14, 16, L4)
+template <int KM, typename TR>
+__device__ __forceinline__ void dpp_mode(TR (&a)[2][5], TR c0, TR c1, TR c2, TR c3, TR c4, TR t0, TR t1);
+#define FVDG_DPP_SPEC(KM, TR, OP, L0, L1, L2, L3, L4)                                                          \
+    template <>                                                                                             \
+    __device__ __forceinline__ void dpp_mode<KM, TR>(TR(&a)[2][5], TR c0, TR c1, TR c2, TR c3, TR c4, TR t0, TR t1) { \
+        asm volatile(FVDG_DPP_BODY(OP, L0, L1, L2, L3, L4)                                                  \
+                     : "+v"(a[0][0]), "+v"(a[0][1]), "+v"(a[0][2]), "+v"(a[0][3]), "+v"(a[0][4]), "+v"(a[1][0]),  \
+                       "+v"(a[1][1]), "+v"(a[1][2]), "+v"(a[1][3]), "+v"(a[1][4])                           \
+                     : "v"(c0), "v"(c1), "v"(c2), "v"(c3), "v"(c4), "v"(t0), "v"(t1));                      \
+    }
+#define FVDG_DPP_SPEC2(KM, L0, L1, L2, L3, L4)                            \
+    FVDG_DPP_SPEC(KM, double, "v_fmac_f64_dpp", L0, L1, L2, L3, L4) \
+    FVDG_DPP_SPEC(KM, float, "v_fmac_f32_dpp", L0, L1, L2, L3, L4)
+FVDG_DPP_SPEC2(0, 0, 1, 2, 3, 4)
+FVDG_DPP_SPEC2(1, 5, 6, 7, 8, 9)
+FVDG_DPP_SPEC2(2, 10, 11, 12, 13, 14)
+FVDG_DPP_SPEC2(3, 15, 0, 1, 2, 3)
+FVDG_DPP_SPEC2(4, 4, 5, 6, 7, 8)
+FVDG_DPP_SPEC2(5, 9, 10, 11, 12, 13)
+FVDG_DPP_SPEC2(6, 14, 15, 0, 1, 2)
+FVDG_DPP_SPEC2(7, 3, 4, 5, 6, 7)
+FVDG_DPP_SPEC2(8, 8, 9, 10, 11, 12)
+FVDG_DPP_SPEC2(9, 13, 14, 15, 0, 1)
+FVDG_DPP_SPEC2(10, 2, 3, 4, 5, 6)
+FVDG_DPP_SPEC2(11, 7, 8, 9, 10, 11)
+FVDG_DPP_SPEC2(12, 12, 13, 14, 15, 0)
+FVDG_DPP_SPEC2(13, 1, 2, 3, 4, 5)
+FVDG_DPP_SPEC2(14, 6, 7, 8, 9, 10)
+FVDG_DPP_SPEC2(15, 11, 12, 13, 14, 15)
+#undef FVDG_DPP_SPEC2
+#undef FVDG_DPP_SPEC
+#undef FVDG_DPP_BODY
+#undef FVDG_DPP_I
+
+// k_face_flux_dpp<ORDER,T>: the trace arithmetic is T.  T = double: FP64 traces (the drop-in).
+// T = float (mixed precision): the coefficients of modes >= 1 and the tables in FP32, FP32
+// accumulation, the cell means (mode 0, phi_0 = 1) added in FP64; Riemann flux in FP64.
 template <int ORDER, typename T = double>
-__global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_face_flux_dpp(FaceFluxArgs<T> a) {
-    using SH = FaceFluxDppShape<ORDER>;
+__global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER, T>::MIN_WAVES)) void k_face_flux_dpp(FaceFluxArgs<T> a) {
+    using SH = FaceFluxDppShape<ORDER, T>;
     constexpr int N = SH::N, QF = SH::QF, NV = SH::NV, GP = SH::GP, LPF = SH::LPF, CR = SH::CR, FB = SH::FB;
     constexpr int NT = SH::THREADS, QS = QF | 1;
-    static_assert(SH::SUPPORTED, "face points exceed one 16-lane row");
-    __shared__ double sPhi[8 * N * QS];  // [table][mode][point], odd point stride
+    static_assert(SH::SUPPORTED && GP == 2, "two face points per lane of a 16-lane row");
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
+    __shared__ T sPhi[8 * N * QS];  // [table][mode][point], odd point stride
     __shared__ int sPerm[3 * QF];
     __shared__ int sCanon[24];
     const int t = threadIdx.x;
     for (int i = t; i < 8 * QF * N; i += NT) {
         const int c = i / (QF * N), g = (i / N) % QF, kk = i % N;
-        sPhi[(c * N + kk) * QS + g] = a.PhiC64[i];
+        sPhi[(c * N + kk) * QS + g] = a.PhiC[i];
     }
     for (int i = t; i < 3 * QF; i += NT) sPerm[i] = a.perm[i];
     if (t < 24) sCanon[t] = a.canon[t];
     __syncthreads();
     const int row = t >> 4, m = t & 15;
-    constexpr bool SPLIT = !std::is_same<T, double>::value;
 
-    for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-        const int64_t f = tile * FB + row;
-        int kind = 0, L = -1, R = -1, cL = 0, cR = 0;
-        if (f < a.F) {
+    // the face of this row in a tile: its kind (0 none, 1 interior, 2 Dirichlet / Pseudo3DZ boundary),
+    // canonical tables of both sides and the raw FP64 coefficients (lane m holds entries m, m + 16,
+    // ... of each block).  The next tile's are loaded while the current tile's Riemann solves run.
+    int64_t f = 0;
+    int kind = 0, cL = 0, cR = 0;
+    double ql[CR], qr[CR];
+    auto fetch = [&](int64_t tile) __attribute__((always_inline)) {
+        f = tile * FB + row;
+        kind = 0;
+        cL = cR = 0;
+        int64_t L = -1, R = -1;
+        if (tile < a.ntiles && f < a.F) {
             L = a.fcell[2 * f];
             R = a.fcell[2 * f + 1];
             cL = sCanon[a.fcombo[2 * f]];
@@ -819,85 +882,115 @@ __global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_f
                 kind = 2;
         }
         const int64_t Lc = kind != 0 ? L : -1, Rc = kind == 1 ? R : -1;
-        double cl[CR], cr[CR];
 #pragma unroll
         for (int r = 0; r < CR; ++r) {
             const int i = r * 16 + m;
-            cl[r] = (Lc >= 0 && i < NV) ? a.U[Lc * NV + i] : 0.0;
-            cr[r] = (Rc >= 0 && i < NV) ? a.U[Rc * NV + i] : 0.0;
+            ql[r] = (Lc >= 0 && i < NV) ? a.U[Lc * NV + i] : 0.0;
+            qr[r] = (Rc >= 0 && i < NV) ? a.U[Rc * NV + i] : 0.0;
         }
-        const double* PL = sPhi + (cL / 3) * N * QS;
-        const double* PR = sPhi + (cR / 3) * N * QS;
+    };
+    if constexpr (SH::PREFETCH) fetch(blockIdx.x);
+    for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        if constexpr (!SH::PREFETCH) fetch(tile);
+        const int64_t fc = f;
+        const int kc = kind;
+        const bool on = fc < a.F && m < LPF;
+        // this tile's face geometry and weights, loaded before the next tile's prefetch
+        double n[3] = {0, 0, 0}, J = 0, wg[GP];
+        if (on) {
+            n[0] = a.fgeo[4 * fc];
+            n[1] = a.fgeo[4 * fc + 1];
+            n[2] = a.fgeo[4 * fc + 2];
+            J = a.fgeo[4 * fc + 3];
+        }
+#pragma unroll
+        for (int i = 0; i < GP; ++i) wg[i] = on ? a.wf[min(m * GP + i, QF - 1)] : 0.0;
+        // split: mode 0 (the cell means, entries 0..4 in register 0 of lanes 0..4) in FP64 to every
+        // lane of the row, the rest rounded to FP32
+        double c0l[5] = {0, 0, 0, 0, 0}, c0r[5] = {0, 0, 0, 0, 0};
+        if constexpr (SPLIT) {
+            static_for<5>([&](auto VC) __attribute__((always_inline)) {
+                constexpr int v = decltype(VC)::value;
+                c0l[v] = row_bcast<v>(ql[0]);
+                c0r[v] = row_bcast<v>(qr[0]);
+            });
+        }
+        T cl[CR], cr[CR];
+#pragma unroll
+        for (int r = 0; r < CR; ++r) {
+            const bool keep = !SPLIT || r * 16 + m >= 5;
+            cl[r] = keep ? (T)ql[r] : (T)0;
+            cr[r] = keep ? (T)qr[r] : (T)0;
+        }
+        const T* PL = sPhi + (cL / 3) * N * QS;
+        const T* PR = sPhi + (cR / 3) * N * QS;
         int gl[GP], gr[GP];
-        double UL[GP][5], UR[GP][5], pl[GP], pr[GP];
+        T UL[GP][5], UR[GP][5], pl[GP], pr[GP];
 #pragma unroll
         for (int i = 0; i < GP; ++i) {
             const int g = min(m * GP + i, QF - 1);
             gl[i] = sPerm[(cL % 3) * QF + g];
             gr[i] = sPerm[(cR % 3) * QF + g];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) UL[i][v] = UR[i][v] = 0.0;
+            for (int v = 0; v < 5; ++v) UL[i][v] = UR[i][v] = (T)0;
         }
-        // trace sums U_side(g)[v] = sum_k phi_side[k](g) c_side[k][v], entry 5k + v of each block
-        // broadcast from lane (5k + v) % 16, register (5k + v) / 16
-        static_for<NV>([&](auto IC) __attribute__((always_inline)) {
-            constexpr int idx = decltype(IC)::value, k = idx / 5, v = idx % 5;
-            if constexpr (v == 0) {
-                // empty asm statements that redefine the accumulators and this mode's table offsets
-                // and coefficient registers in place: the previous mode's products finish above
-                // them and this mode's table reads and broadcasts cannot be hoisted above them
-                // (hoisting all of them spills); no instruction is emitted
-                constexpr int r0 = 5 * k / 16, r1 = (5 * k + 4) / 16;
+        // trace sums U_side(g)[v] = sum_k phi_side[k](g) c_side[k][v]: entry 5k + v of each block
+        // comes from lane (5k + v) % 16, register (5k + v) / 16, as the DPP operand of the FMA
+        static_for<N>([&](auto KC) __attribute__((always_inline)) {
+            constexpr int k = decltype(KC)::value;
+            if constexpr (!SPLIT || k > 0) {
+                // empty asm statements redefining this mode's table offsets: its table reads cannot
+                // be hoisted above the previous mode's products (hoisting all of them spills)
 #pragma unroll
-                for (int i = 0; i < GP; ++i) {
-                    asm volatile("" : "+v"(gl[i]), "+v"(gr[i]));
-#pragma unroll
-                    for (int w = 0; w < 5; ++w) asm volatile("" : "+v"(UL[i][w]), "+v"(UR[i][w]));
-                }
-                asm volatile("" : "+v"(cl[r0]), "+v"(cr[r0]));
-                if constexpr (r1 != r0) asm volatile("" : "+v"(cl[r1]), "+v"(cr[r1]));
+                for (int i = 0; i < GP; ++i) asm volatile("" : "+v"(gl[i]), "+v"(gr[i]));
 #pragma unroll
                 for (int i = 0; i < GP; ++i) {
                     pl[i] = PL[k * QS + gl[i]];
                     pr[i] = PR[k * QS + gr[i]];
                 }
-            }
-            const double cb = row_bcast<idx % 16>(cl[idx / 16]);
-            const double db = row_bcast<idx % 16>(cr[idx / 16]);
-#pragma unroll
-            for (int i = 0; i < GP; ++i) {
-                UL[i][v] = fma(pl[i], cb, UL[i][v]);
-                UR[i][v] = fma(pr[i], db, UR[i][v]);
+                constexpr int r0 = (5 * k) / 16, r1 = (5 * k + 1) / 16, r2 = (5 * k + 2) / 16;
+                constexpr int r3 = (5 * k + 3) / 16, r4 = (5 * k + 4) / 16;
+                dpp_mode<k % 16, T>(UL, cl[r0], cl[r1], cl[r2], cl[r3], cl[r4], pl[0], pl[1]);
+                dpp_mode<k % 16, T>(UR, cr[r0], cr[r1], cr[r2], cr[r3], cr[r4], pr[0], pr[1]);
             }
         });
+        // the next tile's face and coefficients (into ql / qr, free now)
+        if constexpr (SH::PREFETCH) fetch(tile + gridDim.x);
         double fo[GP][5];
 #pragma unroll
         for (int i = 0; i < GP; ++i)
 #pragma unroll
             for (int v = 0; v < 5; ++v) fo[i][v] = 0.0;
-        if (f < a.F && m < LPF) {
-            const double n[3] = {a.fgeo[4 * f], a.fgeo[4 * f + 1], a.fgeo[4 * f + 2]};
+        if (on) {
 #pragma unroll
             for (int i = 0; i < GP; ++i) {
                 const int g = m * GP + i;
                 if (g >= QF) continue;
+                double uL[5], Ub[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    uL[v] = c0l[v] + (double)UL[i][v];
+                    Ub[v] = c0r[v] + (double)UR[i][v];
+                }
                 // outer state: the neighbour's trace, or the ghost state of a Dirichlet (2 U_D - U)
                 // or Pseudo3DZ (mirrored z momentum) face; one Riemann solve per point
-                double Ub[5];
+                if (kc == 2) {
+                    if (a.fbc[fc] == FVDG_BC_DIRICHLET) {
+                        const double* ud = a.UD + ((int64_t)a.fdir[fc] * QF + g) * 5;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) Ub[v] = UR[i][v];
-                if (kind == 2) {
-                    if (a.fbc[f] == FVDG_BC_DIRICHLET) {
-                        const double* ud = a.UD + ((int64_t)a.fdir[f] * QF + g) * 5;
-#pragma unroll
-                        for (int v = 0; v < 5; ++v) Ub[v] = ud[v] + (ud[v] - UL[i][v]);
+                        for (int v = 0; v < 5; ++v) Ub[v] = ud[v] + (ud[v] - uL[v]);
                     } else {
 #pragma unroll
-                        for (int v = 0; v < 5; ++v) Ub[v] = UL[i][v];
-                        Ub[3] = -UL[i][3];
+                        for (int v = 0; v < 5; ++v) Ub[v] = uL[v];
+                        Ub[3] = -uL[3];
                     }
                 }
-                if (kind != 0) num_flux(a.flux, UL[i], Ub, n, a.gamma, fo[i]);
+                if (kc != 0) {
+                    if (SPLIT)
+                        num_flux_mx(a.flux, uL, Ub, n, a.gamma, fo[i]);
+                    else
+                        num_flux(a.flux, uL, Ub, n, a.gamma, fo[i]);
+                }
             }
         }
         // FP32 output: every lane of the row takes point 0's flux from lane 0 (all lanes active)
@@ -906,17 +999,16 @@ __global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER>::MIN_WAVES)) void k_f
 #pragma unroll
             for (int v = 0; v < 5; ++v) ref[v] = row_bcast<0>(fo[0][v]);
         }
-        if (f < a.F && m < LPF) {
-            const double J = a.fgeo[4 * f + 3];
+        if (on) {
             if (SPLIT && m == 0)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) a.Fref[f * 5 + v] = ref[v] * J;
+                for (int v = 0; v < 5; ++v) a.Fref[fc * 5 + v] = ref[v] * J;
 #pragma unroll
             for (int i = 0; i < GP; ++i) {
                 const int g = m * GP + i;
                 if (g >= QF) continue;
-                const double sc = a.wf[g] * J;
-                T* o = a.Fh + (f * QF + g) * 5;
+                const double sc = wg[i] * J;
+                T* o = a.Fh + (fc * QF + g) * 5;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) o[v] = (T)((fo[i][v] - ref[v]) * sc);
             }

@@ -30,6 +30,7 @@
 #include "fvdg_limiter.hpp"
 #include "fvdg_diagnostics.hpp"
 #include "fvdg_bf16x3.hpp"
+#include "fvdg_elem.hpp"
 
 using namespace fvdg;
 
@@ -383,11 +384,11 @@ void launch_surf_fast(fvdg_ctx* c, const SurfArgs& a, bool fuse) {
         hipLaunchKernelGGL((k_surf_lds<P, false>), dim3(grid), dim3(NT), 0, c->stream, A);
 }
 
+// the face-flux pass: every face's traces and numerical flux into Fh (mixed: + Fref)
 template <int P, typename T>
-void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, double* Rout, const StageUpdate* up,
-                    const T* PhiC, T* Fh, double* ms_flux, double* ms_test) {
+void flux_pass(fvdg_ctx* c, const double* U, const T* PhiC, T* Fh, double* ms_flux) {
     FaceFluxArgs<T> f{c->F, c->gamma, c->flux, U, c->fcell.p, c->fcombo.p, c->fgeo.p, c->fbc.p, c->fdir.p,
-                      c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0, c->tPhiC.p};
+                      c->UD.p, c->tWf.p, PhiC, c->tCanon.p, c->tPerm.p, Fh, c->Fref.p, 0};
     constexpr int FB = FaceFluxShape<P, T>::FB;
     f.ntiles = (c->F + FB - 1) / FB;
     // FP64: exactly the resident blocks (measured best at P3: 0.351 ms vs 0.364 with 8 per CU);
@@ -396,22 +397,28 @@ void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, d
     if constexpr (std::is_same<T, double>::value)
         per_cu = resident_blocks<k_face_flux_lds<P, T>, FaceFluxShape<P, T>::THREADS>();
     Span s1(ms_flux, c->stream);
-    if constexpr (P >= 2 && FaceFluxDppShape<P>::SUPPORTED) {
+    if constexpr (P >= 2 && FaceFluxDppShape<P, T>::SUPPORTED) {
         if (!c->lds_face_flux) {
             // P2-P4: one face per 16-lane row, coefficients broadcast from registers, FP64 traces
             // and fluxes (FP32 differences out in the mixed stage)
-            constexpr int FBD = FaceFluxDppShape<P>::FB, NTD = FaceFluxDppShape<P>::THREADS;
+            constexpr int FBD = FaceFluxDppShape<P, T>::FB, NTD = FaceFluxDppShape<P, T>::THREADS;
             f.ntiles = (c->F + FBD - 1) / FBD;
             const int64_t res = resident_blocks<k_face_flux_dpp<P, T>, NTD>();
             const unsigned gd = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * res);
             hipLaunchKernelGGL((k_face_flux_dpp<P, T>), dim3(gd), dim3(NTD), 0, c->stream, f);
         }
     }
-    if (P < 2 || !FaceFluxDppShape<P>::SUPPORTED || c->lds_face_flux) {
+    if (P < 2 || !FaceFluxDppShape<P, T>::SUPPORTED || c->lds_face_flux) {
         const unsigned g1 = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * per_cu);
         hipLaunchKernelGGL((k_face_flux_lds<P, T>), dim3(g1), dim3(FaceFluxShape<P, T>::THREADS), 0, c->stream, f);
     }
     s1.stop();
+}
+
+template <int P, typename T>
+void faces_two_pass(fvdg_ctx* c, const double* U, int parts, const double* Rv, double* Rout, const StageUpdate* up,
+                    const T* PhiC, T* Fh, double* ms_flux, double* ms_test) {
+    flux_pass<P, T>(c, U, PhiC, Fh, ms_flux);
     constexpr int TE = FaceTestShape<P>::TE;
     FaceTestArgs<T> t{};
     t.E = c->E;
@@ -495,6 +502,75 @@ void faces_fast(fvdg_ctx* c, const double* U, int parts, bool have_vol, const St
     }
 }
 
+// The fused element path: the face-flux pass, then k_elem_mfma (volume + face test + stage
+// update, or the residual of the selected parts into c->R).  ms_flux / ms_elem: event times.
+template <int P, typename T>
+void elem_path_t(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up, const T* PhiC, T* Fh,
+                 const T* VA, const T* GA, const T* PhiB, double* ms_flux, double* ms_elem) {
+    if constexpr (ElemShape<P, T>::SUPPORTED) {
+        if (parts & (FVDG_PART_INTERNALS | FVDG_PART_BOUNDARYS)) flux_pass<P, T>(c, U, PhiC, Fh, ms_flux);
+        ElemArgs<T> a{};
+        a.E = c->E;
+        a.gamma = c->gamma;
+        a.parts = parts;
+        a.U = U;
+        a.R = c->R.p;
+        a.cgeo = c->cgeo.p;
+        a.cfaces = c->cfaces.p;
+        a.cmeta = c->cmeta.p;
+        a.VA = VA;
+        a.GA = GA;
+        a.wq = c->tWqa.p;
+        a.Gs = c->tGs.p;
+        a.Fh = Fh;
+        a.Fref = c->Fref.p;
+        a.IC = c->tIC.p;
+        a.PhiB = PhiB;
+        a.canon = c->tCanon.p;
+        a.perm = c->tPerm.p;
+        a.mhat = c->tMhat.p;
+        a.edt = c->edt.n ? c->edt.p : nullptr;
+        a.bad = c->bad.p;
+        if (up) {
+            a.U0 = up->U0;
+            a.out = up->out;
+            a.a0 = up->a0;
+            a.b0 = up->b0;
+            a.dt = up->dt;
+        }
+        Span se(ms_elem, c->stream);
+        if (up)
+            hipLaunchKernelGGL((k_elem_mfma<P, T, true>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_elem_mfma<P, T, false>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, a);
+        se.stop();
+    }
+}
+template <int P>
+void elem_path_p(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up, double* ms_flux, double* ms_elem) {
+    if (mixed(c))
+        elem_path_t<P, float>(c, U, parts, up, c->tPhiC32.p, c->Fh32.p, c->tVA32.p, c->tGA32.p, c->tPhiB32.p,
+                              ms_flux, ms_elem);
+    else
+        elem_path_t<P, double>(c, U, parts, up, c->tPhiC.p, c->Fh.p, c->tVA.p, c->tGA.p, c->tPhiB.p, ms_flux,
+                               ms_elem);
+}
+// the fused element path serves P0-P4 in FP64 and FP32 (BF16x3, P5, the alternative face kernels,
+// the plain kernels and FVDG_ELEM=split take the volume kernel + face-test kernel path)
+bool use_elem(const fvdg_ctx* c) {
+    return c->elem_fused && !c->simple && !c->owner_faces && !c->lds_face_test && c->order <= 4 &&
+           c->precision != FVDG_PREC_BF16X3;
+}
+void elem_path(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up, double* ms_flux, double* ms_elem) {
+    switch (c->order) {
+        case 0: elem_path_p<0>(c, U, parts, up, ms_flux, ms_elem); break;
+        case 1: elem_path_p<1>(c, U, parts, up, ms_flux, ms_elem); break;
+        case 2: elem_path_p<2>(c, U, parts, up, ms_flux, ms_elem); break;
+        case 3: elem_path_p<3>(c, U, parts, up, ms_flux, ms_elem); break;
+        case 4: elem_path_p<4>(c, U, parts, up, ms_flux, ms_elem); break;
+    }
+}
+
 // Face part of the residual of U: added to the volume part (have_vol) and written to
 // c->R, or, with `up`, fed to the SSP-RK stage update.  ms_flux / ms_test (nullable):
 // event-timed face-flux and face-test(+update) kernels; the one-pass owner-computes
@@ -527,6 +603,10 @@ int launch_residual(fvdg_ctx* c, const double* U, int parts, double* ms_vol, dou
         if (parts & (FVDG_PART_INTERNALS | FVDG_PART_BOUNDARYS))
             hipLaunchKernelGGL(k_face_flux, dim3(nblocks(c->F * T.Qf, bs)), dim3(bs), 0, c->stream, a);
         hipLaunchKernelGGL(k_face_test, dim3(nblocks(c->E * T.N, bs)), dim3(bs), 0, c->stream, a);
+        ss.stop();
+    } else if (use_elem(c)) {
+        Span ss(ms_surf, c->stream);
+        elem_path(c, U, parts, nullptr, nullptr, nullptr);
         ss.stop();
     } else {
         const bool vol = (parts & FVDG_PART_CELLS) != 0;
@@ -602,6 +682,9 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
                                dst[s], c->cgeo.p, c->tMhat.p, c->E, c->tab.N, A[s], B[s], dt,
                                c->edt.n ? c->edt.p : nullptr, c->bad.p);
             su.stop();
+        } else if (use_elem(c) && !fused_extrema) {
+            const StageUpdate up{c->U.p, dst[s], A[s], B[s], dt};
+            elem_path(c, src[s], FVDG_PART_ALL, &up, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
         } else {
             Span sv(ms ? &ms[1] : nullptr, c->stream);
             dispatch_vol(c, src[s], fused_extrema ? c->lim_ext.p : nullptr);
@@ -619,6 +702,25 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
     return FVDG_OK;
 }
 
+
+// Names of the kernels one SSP-RK3 stage launches on the context's current scheme, as
+// "volume=<name>;flux=<name>;test=<name>" (an empty name: that slot launches nothing)
+std::string stage_kernel_names(const fvdg_ctx* c) {
+    const int P = c->order;
+    const bool mx = mixed(c);
+    const std::string T = mx ? "float" : "double", ps = std::to_string(P);
+    if (c->simple) return "volume=k_vol_flux+k_vol_test;flux=k_face_flux;test=k_face_test+k_rk_update";
+    const bool fused_extrema = (c->lim_parts & FVDG_LIMIT_BOUNDS) && !c->lim_avg && !mx;
+    const bool dpp = P >= 2 && P <= 4 && !c->lds_face_flux;
+    const std::string flux = (dpp ? "k_face_flux_dpp<" : "k_face_flux_lds<") + ps + "," + T + ">";
+    if (use_elem(c) && !fused_extrema) return "volume=;flux=" + flux + ";test=k_elem_mfma<" + ps + "," + T + ",true>";
+    std::string vol = c->precision == FVDG_PREC_BF16X3 ? "k_vol_bf16x3<" + ps + ">"
+                                                       : "k_vol_mfma<" + ps + "," + T + (fused_extrema ? ",true>" : ">");
+    if (c->owner_faces && !mx) return "volume=" + vol + ";flux=k_surf_lds<" + ps + ",true>;test=";
+    const bool mfma_test = !c->lds_face_test && P <= (mx ? 3 : 4);
+    const std::string test = (mfma_test ? "k_face_test_mfma<" : "k_face_test_lds<") + ps + ",true," + T + ">";
+    return "volume=" + vol + ";flux=" + flux + ";test=" + test;
+}
 
 // Waits for the stream, then reports the stage-output flag the update kernels set
 // (FVDG_ERR_NONFINITE; the state is left as computed).
@@ -690,6 +792,8 @@ int fvdg_create(fvdg_ctx** out, int device) {
     c->owner_faces = faces && std::strcmp(faces, "owner") == 0;
     const char* ftest = std::getenv("FVDG_FACE_TEST");
     c->lds_face_test = ftest && std::strcmp(ftest, "lds") == 0;
+    const char* elem = std::getenv("FVDG_ELEM");
+    c->elem_fused = !(elem && std::strcmp(elem, "split") == 0);
     const char* fflux = std::getenv("FVDG_FACE_FLUX");
     c->lds_face_flux = fflux && std::strcmp(fflux, "lds") == 0;
     if (const char* gm = std::getenv("FVDG_GRID_MULT")) c->grid_mult = std::max(1, std::atoi(gm));
@@ -1085,6 +1189,16 @@ static int time_stages_with(fvdg_ctx* c, double dt, int n_stages, double* ms, hi
         if (rc) return rc;
     }
     for (int i = 1; i < 4; ++i) ms[i] = split[i] / n_stages;
+    return FVDG_OK;
+}
+
+int fvdg_stage_kernels(fvdg_ctx* c, char* buf, int len) {
+    int rc = check_ready(c, false);
+    if (rc) return rc;
+    if (!buf || len <= 0) return fail(FVDG_ERR_ARG, "null or empty buffer");
+    const std::string n = stage_kernel_names(c);
+    if ((int)n.size() >= len) return fail(FVDG_ERR_ARG, "buffer too small for the kernel names");
+    std::memcpy(buf, n.c_str(), n.size() + 1);
     return FVDG_OK;
 }
 

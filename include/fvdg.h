@@ -179,9 +179,14 @@ int fvdg_max_wave_speed(fvdg_ctx* c, double* out);
 int fvdg_conservation_sums(fvdg_ctx* c, double out[5]);
 /* Time n_stages SSP-RK3 stages with HIP events on the solver stream: ms[0] = wall time per
  * stage of a pass with no host synchronisation inside; then a second pass with events between
- * kernels gives ms[1] = volume, ms[2] = face flux, ms[3] = face test + stage update
- * (per-stage averages). */
+ * kernels gives ms[1] = volume, ms[2] = face flux, ms[3] = face test + stage update (or the fused
+ * element kernel: volume + face test + update, with ms[1] = 0) (per-stage averages).  The state
+ * advances by 2 n_stages / 3 steps. */
 int fvdg_time_stages(fvdg_ctx* c, double dt, int n_stages, double* ms);
+/* The kernels one stage launches on the current scheme, "volume=<name>;flux=<name>;test=<name>"
+ * (NUL-terminated into buf[len]; an empty name: nothing is launched in that slot, e.g. the volume
+ * term inside the fused element kernel).  Measurement aid; no reference counterpart. */
+int fvdg_stage_kernels(fvdg_ctx* c, char* buf, int len);
 
 /* ---------------- implicit path: ImplicitConvection<Order,Flux>::assemble
  * (include/DG/DG_Schemes/ImplicitConvection.h:10-54, src/DG/DG_Schemes/ImplicitConvection.cpp)

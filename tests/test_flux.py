@@ -153,3 +153,28 @@ def test_roe_non_physical_state_takes_lax_friedrichs(oracle, bad):
     # physical states keep the Roe flux
     UR2 = np.array([0.6, 0.1, 1.0, 0.0, 2.0])
     assert not np.allclose(oracle.numflux("roe", UL, UR2, n), oracle.numflux("lf", UL, UR2, n))
+
+
+def test_lf_dissipation_sign_and_size(oracle):
+    """The Lax-Friedrichs dissipation against the jump: (U_L - U_R) . (Fhat - (F_L + F_R) n / 2)
+    is positive and lies between 1/2 max(|u.n| + c) |U_L - U_R|^2 and 1/2 max(|u| + c) |U_L - U_R|^2
+    (the sound speeds from the states by numpy): a flipped sign, a missing 1/2 or a normal-speed
+    lambda fails one of the bounds."""
+    g = 1.4
+    S = random_states(40, 9)
+    N = random_normals(20, 10)
+
+    def speeds(U, n):
+        rho, m, E = U[0], U[1:4], U[4]
+        u = m / rho
+        c = np.sqrt(g * (g - 1) * (E - 0.5 * rho * u @ u) / rho)
+        return abs(u @ n) + c, np.linalg.norm(u) + c
+
+    for i in range(20):
+        UL, UR, n = S[i], S[20 + i], N[i]
+        d = UL - UR
+        diss = d @ (oracle.numflux("lf", UL, UR, n) - 0.5 * (oracle.flux(UL) @ n + oracle.flux(UR) @ n))
+        (aL, bL), (aR, bR) = speeds(UL, n), speeds(UR, n)
+        lo, hi = 0.5 * max(aL, aR) * (d @ d), 0.5 * max(bL, bR) * (d @ d)
+        assert diss > 0
+        assert lo * (1 - 1e-12) <= diss <= hi * (1 + 1e-12)
