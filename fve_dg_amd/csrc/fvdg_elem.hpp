@@ -61,8 +61,10 @@ struct ElemArgs {
 template <int ORDER, typename T>
 struct ElemShape {
     static constexpr int N = Ord<ORDER>::N;
-    // two waves per SIMD: the tile's LDS (16 x 5N doubles, 12.9 KB at P3) allows about that many
-    static constexpr int MIN_WAVES = 2;
+    // waves per SIMD the registers are held to: the tile's LDS (16 x 5N doubles, 12.9 KB at P3) allows
+    // three; the FP64 form would spill 456 bytes per lane at three (two), the FP32 form 20 (three:
+    // P3 196k 0.434 vs 0.457 ms)
+    static constexpr int MIN_WAVES = (N <= 20 && !std::is_same<T, double>::value) ? 3 : 2;
     static constexpr bool SUPPORTED = N <= 35;
 };
 
@@ -76,8 +78,6 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
     constexpr int KPAD = S::KPAD, JPAD = S::JPAD, STR = S::STRIDE;
     constexpr int QF = Ord<ORDER>::QF, FKS = (QF + 3) / 4;
     __shared__ double sU[16 * STR];  // the tile's coefficients (FP64), later its stage values (or residual)
-    __shared__ double sRef[SPLIT ? 16 * 4 * 5 : 1];
-    __shared__ int sSlotC[SPLIT ? 16 * 4 : 1];
     __shared__ int sPinv[3 * QF];
     __shared__ int sCanon[24];
     const int l = threadIdx.x, col = l & 15, grp = l >> 4;
@@ -190,6 +190,8 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
         cf4 = *reinterpret_cast<const int4*>(a.cfaces + e * 4);
         cm4 = *reinterpret_cast<const uchar4*>(a.cmeta + e * 4);
     }
+    int slot_c[4] = {-1, -1, -1, -1};  // split: each slot's table and face (-f-1: side 1), for step 5
+    int64_t slot_f[4] = {0, 0, 0, 0};
 #pragma unroll 1
     for (int s = 0; s < 4; ++s) {
         int cl = -1, m = 0;
@@ -207,10 +209,9 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                 sign = (slot & 1) ? -1 : 1;
             }
         }
-        if (SPLIT && grp == 0) {
-            sSlotC[col * 4 + s] = cl;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) sRef[(col * 4 + s) * 5 + v] = cl >= 0 ? (double)sign * a.Fref[f * 5 + v] : 0.0;
+        if (SPLIT) {
+            slot_c[s] = cl;
+            slot_f[s] = sign < 0 ? -(f + 1) : f;
         }
         unsigned used = 0;
 #pragma unroll
@@ -248,15 +249,16 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
     // 5. lane (j = 16 jt + row(r, grp), element col): R_v[j] from the accumulators, plus (split) the
     //    mean flux's volume term and the faces' reference fluxes in FP64; FUSE: the stage value
     //    U + dt (-R / M) replaces U in the LDS tile (the same lane reads and writes the entry)
-    __syncthreads();  // the slots' reference fluxes are in LDS
     double fref[4][5];
     int fc[4] = {-1, -1, -1, -1};
     if (SPLIT) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            fc[s] = sSlotC[col * 4 + s];
+            fc[s] = slot_c[s];
+            const int64_t fs = slot_f[s] < 0 ? -slot_f[s] - 1 : slot_f[s];
+            const double sg = slot_f[s] < 0 ? -1.0 : 1.0;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) fref[s][v] = sRef[(col * 4 + s) * 5 + v];
+            for (int v = 0; v < 5; ++v) fref[s][v] = fc[s] >= 0 ? sg * a.Fref[fs * 5 + v] : 0.0;
         }
     }
     const double dte = FUSE ? (a.edt ? (e < a.E ? a.edt[e] : 0.0) : a.dt) : 0.0;
