@@ -246,3 +246,53 @@ def test_gpu_stage_limiter_keeps_a_rarefaction_run_physical(solver, oracle):
     assert np.isfinite(V).all()
     assert V[..., 0].min() >= 0 and pressure(V).min() >= -1e-12
     assert V[..., 0].min() < 0.6  # the rarefactions opened a low-density region
+
+
+def _oracle_limited_rarefaction(oracle, speed, steps):
+    """The double rarefaction (rho 1, p 0.4, u = -+speed at x = 5) on 16 x 2 x 2 x 6 periodic tets,
+    P2, Lax-Friedrichs, advanced by the oracle's SSP-RK3 with PositiveLimiter::apply_2 after every
+    stage (bounds from the stage input), dt = 0.1 h_min / (5 lambda_0).  Returns the number of steps
+    done and whether the state stayed finite."""
+    order = 2
+    mesh = F.Mesh.box((16, 2, 2), ub=(10, 1.25, 1.25), periodic=7)
+    m = mesh.export()
+    og = oracle.geometry(m)
+    vol = np.abs(og["det"]) / 6
+    area = np.zeros(len(vol))
+    np.add.at(area, m["face_cells"][:, 0], og["area"])
+    inner = m["face_cells"][:, 1] >= 0
+    np.add.at(area, m["face_cells"][inner, 1], og["area"][inner])
+    h_min = 3 * (vol / area).min()
+    xc = m["points"][m["cells"]].mean(axis=1)[:, 0]
+    U = np.zeros((len(xc), 5 * F.num_basis(order)))
+    u = np.where(xc < 5, -speed, speed)
+    U[:, 0], U[:, 1], U[:, 4] = 1.0, u, 0.4 / 0.4 + 0.5 * u * u
+    dt = 0.1 * h_min / (5 * (speed + np.sqrt(GAMMA * 0.4)))
+    M = oracle.mass(m, order)
+
+    def L(V):
+        return -oracle.residual(m, order, V, flux="lf", threads=oracle.max_threads()) / M
+
+    def lim(V, Vin):
+        return oracle.limiter_apply(order, V, oracle.limiter_bounds(m, order, Vin), parts=2)
+
+    for n in range(steps):
+        U1 = lim(U + dt * L(U), U)
+        U2 = lim(0.75 * U + 0.25 * (U1 + dt * L(U1)), U1)
+        U = lim(U / 3 + 2 / 3 * (U2 + dt * L(U2)), U2)
+        if not np.isfinite(U).all():
+            return n + 1, False
+    return steps, True
+
+
+def test_reference_form_limiter_near_vacuum(oracle):
+    """Finding about the reference's limiter, shown on the CPU with the oracle: apply_2 samples the
+    volume quadrature points only (PositiveLimiter.h:171-259, eps = 1e-16), so the limited scheme is
+    not positivity-preserving at the face points the fluxes use.  With u = -+1 (star density ~0.21)
+    the limited run stays finite; with u = -+2 (near vacuum between the rarefactions) it turns
+    non-finite within 60 steps -- as the GPU run of the same case did, so that case is not run on the
+    GPU (the stepper reports it as FVDG_ERR_NONFINITE)."""
+    n, finite = _oracle_limited_rarefaction(oracle, 1.0, 60)
+    assert finite and n == 60
+    n, finite = _oracle_limited_rarefaction(oracle, 2.0, 60)
+    assert not finite and n <= 60
