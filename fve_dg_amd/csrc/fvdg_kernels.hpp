@@ -460,6 +460,48 @@ __global__ void k_max_wave_speed(KernelArgs a, double* __restrict__ part) {
 }
 #endif
 
+// max over face points of max(|u|+c) in a grid-stride loop: one partial per block (any mesh size fits
+// a fixed number of partials), then k_max_partials folds them into out[0]
+#ifndef FVDG_DEVICE_HELPERS_ONLY
+__global__ void k_max_wave_speed_gs(KernelArgs a, double* __restrict__ part) {
+    __shared__ double sm[256];
+    const int64_t n = a.F * a.Qf;
+    double m = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = t / a.Qf;
+        const int g = (int)(t - f * a.Qf);
+        double U[5];
+        face_trace(a, a.fcell[2 * f], a.fcombo[2 * f], g, U);
+        m = max_keep_nan(m, wave_speed_checked(U, a.gamma));
+        if (a.fcell[2 * f + 1] >= 0) {
+            face_trace(a, a.fcell[2 * f + 1], a.fcombo[2 * f + 1], g, U);
+            m = max_keep_nan(m, wave_speed_checked(U, a.gamma));
+        }
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = max_keep_nan(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// out[0] = NaN-keeping maximum of part[0..n) (one block of 256 threads)
+__global__ void k_max_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
+    __shared__ double sm[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = max_keep_nan(m, part[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = max_keep_nan(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sm[0];
+}
+#endif
+
 // per face: max over its points of max(|u|+c) of the traces (one thread per face)
 #ifndef FVDG_DEVICE_HELPERS_ONLY
 __global__ void k_face_wave_speed(KernelArgs a, double* __restrict__ out) {

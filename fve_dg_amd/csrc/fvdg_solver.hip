@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "fvdg_device.hpp"
 #include "fvdg_internal.hpp"
@@ -34,7 +35,32 @@
 
 using namespace fvdg;
 
+// Scratch memory of fvdg_stable_dt_noalloc: the partial maxima of the wave-speed reduction
+// (kScratchPartials, one per block of a grid-stride launch) and the folded maximum after them.
+// One copy per GPU for the life of the process (32 KiB); host code uses the per-device address
+// from hipGetSymbolAddress (g_scratch), never the array's name.  Calls that overlap on one GPU
+// share it and race.
+constexpr int kScratchPartials = 4096;
+static __device__ __align__(256) double scratch[kScratchPartials + 1];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// device address of `scratch` on the current device `device` (set once per device)
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(FVDG_ERR_UNSUPPORTED, "device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device]) return FVDG_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(FVDG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<double*>(p);
+    return FVDG_OK;
+}
 
 int check_ready(fvdg_ctx* c, bool need_state) {
     if (!c) return fail(FVDG_ERR_ARG, "null context");
@@ -798,6 +824,8 @@ int fvdg_create(fvdg_ctx** out, int device) {
     c->lds_face_flux = fflux && std::strcmp(fflux, "lds") == 0;
     if (const char* gm = std::getenv("FVDG_GRID_MULT")) c->grid_mult = std::max(1, std::atoi(gm));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const int rc = scratch_init(device);  // the static scratch's address on this device
+    if (rc) return rc;
     *out = c.release();
     return FVDG_OK;
 }
@@ -1038,6 +1066,36 @@ int fvdg_stable_dt(fvdg_ctx* c, double cfl, double* dt) {
     double lam = 0;
     int rc = fvdg_max_wave_speed(c, &lam);
     if (rc) return rc;
+    if (!(lam > 0) || !std::isfinite(lam)) return fail(FVDG_ERR_NONFINITE, "maximum wave speed is not positive and finite");
+    *dt = cfl * c->h_min / ((2 * c->order + 1) * lam);
+    return FVDG_OK;
+}
+
+int fvdg_stable_dt_noalloc(fvdg_ctx* c, double cfl, double* dt) {
+    if (!dt || !(cfl > 0)) return fail(FVDG_ERR_ARG, "bad cfl or null output");
+    int rc = check_ready(c, true);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    double* scr = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch[c->device] : nullptr;
+    if (!scr) return fail(FVDG_ERR_STATE, "scratch address of this device not set (fvdg_create sets it)");
+    KernelArgs a;
+    a.E = c->E;
+    a.F = c->F;
+    a.N = c->tab.N;
+    a.Qf = c->tab.Qf;
+    a.gamma = c->gamma;
+    a.U = c->U.p;
+    a.fcell = c->fcell.p;
+    a.fcombo = c->fcombo.p;
+    a.Phi = c->tPhi.p;
+    const int64_t n = c->F * c->tab.Qf;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(nblocks(n, 256), kScratchPartials));
+    hipLaunchKernelGGL(k_max_wave_speed_gs, dim3(nb), dim3(256), 0, c->stream, a, scr);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, c->stream, scr, nb, scr + kScratchPartials);
+    HIP_TRY(hipGetLastError());
+    double lam = 0;
+    HIP_TRY(hipMemcpyAsync(&lam, scr + kScratchPartials, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     if (!(lam > 0) || !std::isfinite(lam)) return fail(FVDG_ERR_NONFINITE, "maximum wave speed is not positive and finite");
     *dt = cfl * c->h_min / ((2 * c->order + 1) * lam);
     return FVDG_OK;

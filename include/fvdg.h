@@ -172,6 +172,13 @@ int fvdg_ssprk3_stage(fvdg_ctx* c, double dt, int stage);  /* same FVDG_ERR_NONF
  * lambda_max = max over face points of max_{L,R}(|u| + c) of the device state
  * (examples/Sedov/SedovBatch.cpp:174-199 form, with the tet inradius as h) */
 int fvdg_stable_dt(fvdg_ctx* c, double cfl, double* dt);
+/* fvdg_stable_dt without any device allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same result as fvdg_stable_dt.  Its reduction partials live in a
+ * library-static device array of 4,096 + 1 doubles (32 KiB per GPU, held until the process ends;
+ * the grid-stride reduction fits any mesh in it); calls that overlap on one GPU share that array
+ * and race. */
+int fvdg_stable_dt_noalloc(fvdg_ctx* c, double cfl, double* dt);
 /* max over all face quadrature points of max(|u|+c) of the two traces; NaN when any trace is
  * not finite (fvdg_stable_dt then returns FVDG_ERR_NONFINITE) */
 int fvdg_max_wave_speed(fvdg_ctx* c, double* out);
