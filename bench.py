@@ -358,6 +358,9 @@ def run_ensemble(args, w):
             ens.step(args.warmup, graph=True)
         ms = ens.time_steps(args.steps, graph=True)
         res[prec] = dict(ms=ms, value=dofs / (ms / 3 * 1e-3), U=[ens.download(p) for p in range(P)], tuned=tuned)
+        if prec == "fp64":  # the same graph with the order groups one after another on one stream
+            ens.set_serial(True)
+            res[prec]["serial_ms"] = ens.time_steps(args.steps, graph=True)
         ens.close()
         print(f"[bench] {prec}: {ms:.3f} ms/step, {res[prec]['value']:.4g} DOF-updates/s", file=sys.stderr,
               flush=True)
@@ -379,6 +382,8 @@ def run_ensemble(args, w):
         "config": {"workload": args.workload, "problems": P, "tets": tets, "dofs": dofs,
                    "orders": {int(o): int((orders == o).sum()) for o in sorted(set(orders.tolist()))},
                    "hexes_per_axis": [int(ns.min()), int(ns.max())], "stages_per_step": 3, "hip_graph": True,
+                   "concurrent_order_groups": True,
+                   "serial_groups_ms_per_step": round(res["fp64"]["serial_ms"], 4),
                    "autotuned": {str(k): v for k, v in res["fp64"]["tuned"].items()}},
         "roofline": None,
         "mixed": {"precision": "fp32 contractions, fp64 fluxes/update/state", "value": round(res["fp32"]["value"], 1),

@@ -120,6 +120,7 @@ def lib():
         _sig(L, "fvdg_ensemble_step", [_vp, C.c_int, C.c_int])
         _sig(L, "fvdg_ensemble_time_steps", [_vp, C.c_int, C.c_int, _d])
         _sig(L, "fvdg_ensemble_autotune", [_vp, C.c_int, C.POINTER(C.c_int)])
+        _sig(L, "fvdg_ensemble_set_serial", [_vp, C.c_int])
         _lib = L
     return _lib
 
@@ -532,11 +533,17 @@ class Ensemble:
         return ms.value
 
     def autotune(self, trial_steps=2):
+        """per order: {'elem_fused': bool, 'face_flux': 'dpp' | 'lds', 'flux_threads': block size}"""
         ng = len(self.groups())
         ch = (C.c_int * (2 * ng))()
         _check(lib().fvdg_ensemble_autotune(self._h, trial_steps, ch))
-        return {o: {"owner_faces": bool(ch[2 * i]), "blocks_per_cu": ch[2 * i + 1]}
+        return {o: {"elem_fused": bool(ch[2 * i]), "face_flux": "dpp" if ch[2 * i + 1] else "lds",
+                    "flux_threads": ch[2 * i + 1] or None}
                 for i, o in enumerate(self.groups())}
+
+    def set_serial(self, serial=True):
+        """every order group on one stream after another (comparison) instead of concurrent groups"""
+        _check(lib().fvdg_ensemble_set_serial(self._h, int(bool(serial))))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -53,6 +53,7 @@ struct fvdg_ctx {
     hipStream_t stream = nullptr;
     bool owns_stream = true;  // false: a stream lent by an ensemble
     int grid_mult = 8;        // persistent face kernels: blocks per compute unit (tuned by ensembles)
+    int flux_threads = 256;   // block size of the row-broadcast face-flux kernel: 128, 256 or 512 (tuned by ensembles)
     // scheme
     int order = -1, flux = FVDG_FLUX_LF, precision = FVDG_PREC_FP64;
     double gamma = 1.4;

@@ -3,14 +3,17 @@
 //
 // The problems of one order form one disjoint-union mesh (independent connected
 // components), so a single launch of each stage kernel covers all of them; each
-// problem keeps its own time step through a per-element step size.  All order groups
-// share one stream, and one SSP-RK3 step of the whole ensemble (3 stages x the kernels
-// of every group) is captured once into a HIP graph and replayed per step.
+// problem keeps its own time step through a per-element step size.  Each order group
+// has its own stream: one SSP-RK3 step forks from the ensemble's stream to every group
+// (an event), runs the group's three stages (face-flux pass + element kernel each), and
+// joins back, so groups of different orders run concurrently and a small group does not
+// leave the device idle.  One step is captured once into a HIP graph (the fork/join
+// becomes its branches) and replayed per step.
 //
-// The autotuner times, per order group, the face-kernel variants on the ensemble's own
-// data (the two-pass face kernels and the one-pass owner-computes kernel, FP64 only)
-// and the number of persistent blocks per compute unit (2, 4, 8, 16), keeping the
-// fastest; the state is restored after every trial.
+// The autotuner times, per order group on the ensemble's own data, the stage's kernel
+// choices: the fused element kernel or the volume + face-test kernels, the row-broadcast
+// face-flux kernel in 128-, 256- or 512-thread blocks (16 faces per 256 threads) or the
+// LDS-staged one, keeping the fastest; the state is restored after every trial.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +42,9 @@ struct fvdg_ensemble {
     std::vector<double> dt;  // per problem
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool serial = false;  // every group on the ensemble's stream, one after another (comparison)
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -49,6 +55,8 @@ struct fvdg_ensemble {
     ~fvdg_ensemble() {
         drop_graph();
         for (auto& g : groups) fvdg_destroy(g.ctx);
+        if (fork) (void)hipEventDestroy(fork);
+        for (hipEvent_t ev : joins) (void)hipEventDestroy(ev);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -75,11 +83,49 @@ int upload_dt(fvdg_ensemble* e) {
     return FVDG_OK;
 }
 
+// one step of every group: forked from and joined back into the ensemble's stream
 int step_once(fvdg_ensemble* e) {
-    for (auto& g : e->groups) {
-        const int rc = ssprk3_once(g.ctx, 0.0, nullptr);
-        if (rc) return rc;
+    if (e->serial) {
+        for (auto& g : e->groups) {
+            hipStream_t own = g.ctx->stream;
+            g.ctx->stream = e->stream;
+            const int rc = ssprk3_once(g.ctx, 0.0, nullptr);
+            g.ctx->stream = own;
+            if (rc) return rc;
+        }
+        return FVDG_OK;
     }
+    HIP_TRY(hipEventRecord(e->fork, e->stream));
+    for (size_t i = 0; i < e->groups.size(); ++i) {
+        fvdg_ctx* c = e->groups[i].ctx;
+        HIP_TRY(hipStreamWaitEvent(c->stream, e->fork, 0));
+        const int rc = ssprk3_once(c, 0.0, nullptr);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(e->joins[i], c->stream));
+        HIP_TRY(hipStreamWaitEvent(e->stream, e->joins[i], 0));
+    }
+    return FVDG_OK;
+}
+
+// the stage kernels of one group, timed over `steps` steps on its own stream (after one warm step)
+int time_group(fvdg_ctx* c, int steps, float* ms) {
+    int rc = ssprk3_once(c, 0.0, nullptr);
+    if (rc) return rc;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(FVDG_ERR_HIP, "hipEventCreate failed");
+    }
+    hipError_t he = hipEventRecord(a, c->stream);
+    for (int i = 0; i < steps && rc == FVDG_OK && he == hipSuccess; ++i) rc = ssprk3_once(c, 0.0, nullptr);
+    if (he == hipSuccess) he = hipEventRecord(b, c->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(b);
+    if (he == hipSuccess) he = hipEventElapsedTime(ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (he != hipSuccess) return fail(FVDG_ERR_HIP, std::string("group timing: ") + hipGetErrorString(he));
     return FVDG_OK;
 }
 
@@ -95,6 +141,7 @@ int fvdg_ensemble_create(fvdg_ensemble** out, int device, int n, const fvdg_mesh
     e->precision = precision;
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
     std::map<int, std::vector<int>> by_order;
     for (int p = 0; p < n; ++p) {
         if (!meshes[p]) return fail(FVDG_ERR_ARG, "null mesh in ensemble");
@@ -114,12 +161,14 @@ int fvdg_ensemble_create(fvdg_ensemble** out, int device, int n, const fvdg_mesh
         } catch (const std::exception& ex) {
             return fail(FVDG_ERR_MESH, ex.what());
         }
-        int rc = fvdg_create(&g.ctx, device);
+        int rc = fvdg_create(&g.ctx, device);  // the group works on its own stream
         if (rc) return rc;
-        // the group works on the ensemble's stream
-        HIP_TRY(hipStreamDestroy(g.ctx->stream));
-        g.ctx->stream = e->stream;
-        g.ctx->owns_stream = false;
+        hipEvent_t join = nullptr;
+        if (hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
+            fvdg_destroy(g.ctx);
+            return fail(FVDG_ERR_HIP, "hipEventCreateWithFlags failed");
+        }
+        e->joins.push_back(join);
         rc = fvdg_set_scheme(g.ctx, g.order, flux, gamma, precision);
         if (rc == FVDG_OK) rc = fvdg_set_mesh(g.ctx, g.mesh.get());
         if (rc) {
@@ -158,6 +207,7 @@ int fvdg_ensemble_upload(fvdg_ensemble* e, int p, const double* U) {
     if (rc) return rc;
     if (!U) return fail(FVDG_ERR_ARG, "null state");
     auto& g = e->groups[e->prob_group[p]];
+    HIP_TRY(hipStreamSynchronize(g.ctx->stream));
     const int m = e->prob_member[p];
     const size_t NV = 5 * (size_t)g.ctx->tab.N;
     HIP_TRY(hipSetDevice(e->device));
@@ -173,6 +223,7 @@ int fvdg_ensemble_download(fvdg_ensemble* e, int p, double* U) {
     if (rc) return rc;
     if (!U) return fail(FVDG_ERR_ARG, "null state");
     auto& g = e->groups[e->prob_group[p]];
+    HIP_TRY(hipStreamSynchronize(g.ctx->stream));
     const int m = e->prob_member[p];
     const size_t NV = 5 * (size_t)g.ctx->tab.N;
     HIP_TRY(hipSetDevice(e->device));
@@ -261,6 +312,7 @@ int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices) {
     if (!e || trial_steps <= 0) return fail(FVDG_ERR_ARG, "bad arguments");
     HIP_TRY(hipSetDevice(e->device));
     e->drop_graph();
+    HIP_TRY(hipStreamSynchronize(e->stream));
     int gi = 0;
     for (auto& g : e->groups) {
         fvdg_ctx* c = g.ctx;
@@ -268,40 +320,44 @@ int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices) {
         const size_t nd = (size_t)c->E * 5 * c->tab.N;
         DevBuf<double> save;
         HIP_TRY(save.alloc(nd));
-        HIP_TRY(hipMemcpyAsync(save.p, c->U.p, nd * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-        const bool owner_ok = c->precision == FVDG_PREC_FP64;
+        HIP_TRY(hipMemcpyAsync(save.p, c->U.p, nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        // candidates: (fused element kernel, face-flux kernel: 0 = LDS-staged, else row-broadcast block size)
+        const bool dpp_ok = c->order >= 2 && c->order <= 4;
+        const bool elem_ok = c->order <= 4 && c->precision != FVDG_PREC_BF16X3;
         double best = 1e300;
-        int best_owner = 0, best_mult = c->grid_mult;
-        for (int owner = 0; owner <= (owner_ok ? 1 : 0); ++owner)
-            for (int mult : {2, 4, 8, 16}) {
-                c->owner_faces = owner != 0;
-                c->grid_mult = mult;
-                int rc = ssprk3_once(c, 0.0, nullptr);  // warm
-                if (rc) return rc;
-                hipEvent_t a, b;
-                HIP_TRY(hipEventCreate(&a));
-                HIP_TRY(hipEventCreate(&b));
-                HIP_TRY(hipEventRecord(a, e->stream));
-                for (int i = 0; i < trial_steps && rc == FVDG_OK; ++i) rc = ssprk3_once(c, 0.0, nullptr);
-                HIP_TRY(hipEventRecord(b, e->stream));
-                HIP_TRY(hipEventSynchronize(b));
+        int best_elem = c->elem_fused, best_flux = c->lds_face_flux ? 0 : c->flux_threads;
+        for (int elem = elem_ok ? 1 : 0; elem >= 0; --elem)
+            for (int flux : {256, 128, 512, 0}) {
+                if (flux != 0 && !dpp_ok) continue;
+                if (flux == 0 && dpp_ok && elem == 0) continue;  // the split path with the LDS flux: not competitive
+                c->elem_fused = elem != 0;
+                c->lds_face_flux = flux == 0;
+                c->flux_threads = flux != 0 ? flux : 256;
                 float t = 0;
-                HIP_TRY(hipEventElapsedTime(&t, a, b));
-                (void)hipEventDestroy(a);
-                (void)hipEventDestroy(b);
+                int rc = time_group(c, trial_steps, &t);
                 if (rc) return rc;
-                if (t < best) best = t, best_owner = owner, best_mult = mult;
-                HIP_TRY(hipMemcpyAsync(c->U.p, save.p, nd * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+                if (t < best) best = t, best_elem = elem, best_flux = flux;
+                HIP_TRY(hipMemcpyAsync(c->U.p, save.p, nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             }
-        c->owner_faces = best_owner != 0;
-        c->grid_mult = best_mult;
-        HIP_TRY(hipStreamSynchronize(e->stream));
+        c->elem_fused = best_elem != 0;
+        c->lds_face_flux = best_flux == 0;
+        c->flux_threads = best_flux != 0 ? best_flux : 256;
+        HIP_TRY(hipStreamSynchronize(c->stream));
         if (choices) {
-            choices[2 * gi] = best_owner;
-            choices[2 * gi + 1] = best_mult;
+            choices[2 * gi] = best_elem;
+            choices[2 * gi + 1] = best_flux;
         }
         ++gi;
     }
+    return FVDG_OK;
+}
+
+int fvdg_ensemble_set_serial(fvdg_ensemble* e, int serial) {
+    if (!e) return fail(FVDG_ERR_ARG, "null ensemble");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    e->drop_graph();
+    e->serial = serial != 0;
     return FVDG_OK;
 }
 

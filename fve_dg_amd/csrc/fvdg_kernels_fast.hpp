@@ -753,12 +753,14 @@ __global__ __launch_bounds__((FaceFluxShape<ORDER, T>::THREADS), (FaceFluxShape<
 //   k_face_flux_lds; with T = float (the mixed-precision stage) the traces and fluxes stay FP64
 //   and the face's reference flux (its point 0, held by lane 0 of the row) reaches the row by one
 //   more broadcast: Fref = F_0 2A in FP64, Fh[g] = (F_g - F_0) w_g 2A in FP32.
-template <int ORDER, typename T = double>
+template <int ORDER, typename T = double, int NTHREADS = 256>
 struct FaceFluxDppShape {
     static constexpr int N = Ord<ORDER>::N, QF = Ord<ORDER>::QF, NV = 5 * N;
     static constexpr int GP = (QF + 15) / 16, LPF = (QF + GP - 1) / GP;  // points per lane, lanes per face
     static constexpr int CR = (NV + 15) / 16;                           // coefficient registers per side
-    static constexpr int THREADS = 256, FB = THREADS / 16;              // faces per block and tile
+    // faces per block and tile: one per 16-lane row (the ensemble autotuner also tries 128- and
+    // 512-thread blocks: the tables are loaded once per block)
+    static constexpr int THREADS = NTHREADS, FB = THREADS / 16;
     // FP64: registers held to 3 waves per SIMD up to P3 (measured faster than 2 waves; P4's tables,
     // 56 KB, allow 2 blocks per CU anyway), each tile's coefficients loaded at its start.  FP32 traces
     // (mixed): the next tile's face and coefficients are loaded during the current tile's Riemann
@@ -840,9 +842,9 @@ FVDG_DPP_SPEC2(15, 11, 12, 13, 14, 15)
 // k_face_flux_dpp<ORDER,T>: the trace arithmetic is T.  T = double: FP64 traces (the drop-in).
 // T = float (mixed precision): the coefficients of modes >= 1 and the tables in FP32, FP32
 // accumulation, the cell means (mode 0, phi_0 = 1) added in FP64; Riemann flux in FP64.
-template <int ORDER, typename T = double>
-__global__ __launch_bounds__(256, (FaceFluxDppShape<ORDER, T>::MIN_WAVES)) void k_face_flux_dpp(FaceFluxArgs<T> a) {
-    using SH = FaceFluxDppShape<ORDER, T>;
+template <int ORDER, typename T = double, int NTHREADS = 256>
+__global__ __launch_bounds__(NTHREADS, (FaceFluxDppShape<ORDER, T, NTHREADS>::MIN_WAVES)) void k_face_flux_dpp(FaceFluxArgs<T> a) {
+    using SH = FaceFluxDppShape<ORDER, T, NTHREADS>;
     constexpr int N = SH::N, QF = SH::QF, NV = SH::NV, GP = SH::GP, LPF = SH::LPF, CR = SH::CR, FB = SH::FB;
     constexpr int NT = SH::THREADS, QS = QF | 1;
     static_assert(SH::SUPPORTED && GP == 2, "two face points per lane of a 16-lane row");

@@ -410,6 +410,16 @@ void launch_surf_fast(fvdg_ctx* c, const SurfArgs& a, bool fuse) {
         hipLaunchKernelGGL((k_surf_lds<P, false>), dim3(grid), dim3(NT), 0, c->stream, A);
 }
 
+// the row-broadcast face-flux kernel in NT-thread blocks, a persistent grid of the resident blocks
+template <int P, typename T, int NT>
+void launch_flux_dpp(fvdg_ctx* c, FaceFluxArgs<T> f) {
+    constexpr int FBD = FaceFluxDppShape<P, T, NT>::FB;
+    f.ntiles = (c->F + FBD - 1) / FBD;
+    const int64_t res = resident_blocks<k_face_flux_dpp<P, T, NT>, NT>();
+    const unsigned gd = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * res);
+    hipLaunchKernelGGL((k_face_flux_dpp<P, T, NT>), dim3(gd), dim3(NT), 0, c->stream, f);
+}
+
 // the face-flux pass: every face's traces and numerical flux into Fh (mixed: + Fref)
 template <int P, typename T>
 void flux_pass(fvdg_ctx* c, const double* U, const T* PhiC, T* Fh, double* ms_flux) {
@@ -425,13 +435,13 @@ void flux_pass(fvdg_ctx* c, const double* U, const T* PhiC, T* Fh, double* ms_fl
     Span s1(ms_flux, c->stream);
     if constexpr (P >= 2 && FaceFluxDppShape<P, T>::SUPPORTED) {
         if (!c->lds_face_flux) {
-            // P2-P4: one face per 16-lane row, coefficients broadcast from registers, FP64 traces
-            // and fluxes (FP32 differences out in the mixed stage)
-            constexpr int FBD = FaceFluxDppShape<P, T>::FB, NTD = FaceFluxDppShape<P, T>::THREADS;
-            f.ntiles = (c->F + FBD - 1) / FBD;
-            const int64_t res = resident_blocks<k_face_flux_dpp<P, T>, NTD>();
-            const unsigned gd = (unsigned)std::min<int64_t>(f.ntiles, (int64_t)c->n_cu * res);
-            hipLaunchKernelGGL((k_face_flux_dpp<P, T>), dim3(gd), dim3(NTD), 0, c->stream, f);
+            // P2-P4: one face per 16-lane row, coefficients broadcast from registers
+            if (c->flux_threads == 128)
+                launch_flux_dpp<P, T, 128>(c, f);
+            else if (c->flux_threads == 512)
+                launch_flux_dpp<P, T, 512>(c, f);
+            else
+                launch_flux_dpp<P, T, 256>(c, f);
         }
     }
     if (P < 2 || !FaceFluxDppShape<P, T>::SUPPORTED || c->lds_face_flux) {
@@ -738,7 +748,8 @@ std::string stage_kernel_names(const fvdg_ctx* c) {
     if (c->simple) return "volume=k_vol_flux+k_vol_test;flux=k_face_flux;test=k_face_test+k_rk_update";
     const bool fused_extrema = (c->lim_parts & FVDG_LIMIT_BOUNDS) && !c->lim_avg && !mx;
     const bool dpp = P >= 2 && P <= 4 && !c->lds_face_flux;
-    const std::string flux = (dpp ? "k_face_flux_dpp<" : "k_face_flux_lds<") + ps + "," + T + ">";
+    const std::string flux = dpp ? "k_face_flux_dpp<" + ps + "," + T + "," + std::to_string(c->flux_threads) + ">"
+                                 : "k_face_flux_lds<" + ps + "," + T + ">";
     if (use_elem(c) && !fused_extrema) return "volume=;flux=" + flux + ";test=k_elem_mfma<" + ps + "," + T + ",true>";
     std::string vol = c->precision == FVDG_PREC_BF16X3 ? "k_vol_bf16x3<" + ps + ">"
                                                        : "k_vol_mfma<" + ps + "," + T + (fused_extrema ? ",true>" : ">");

@@ -290,10 +290,15 @@ int fvdg_ensemble_set_dt(fvdg_ensemble* e, const double* dt);
 int fvdg_ensemble_step(fvdg_ensemble* e, int n_steps, int use_graph);
 /* average ms per ensemble step over n_steps (after one untimed step) */
 int fvdg_ensemble_time_steps(fvdg_ensemble* e, int n_steps, int use_graph, double* ms_per_step);
-/* per order group, time the face-kernel variant (two-pass / owner-computes, FP64 only) and
- * the persistent blocks per compute unit (2, 4, 8, 16) over trial_steps steps on the
- * ensemble's own state, keep the fastest (state restored); choices[2g] = owner, [2g+1] = blocks */
+/* per order group, time the stage's kernel choices over trial_steps steps on the ensemble's own
+ * state and keep the fastest (state restored): the fused element kernel or the volume + face-test
+ * kernels, and the face-flux kernel (row-broadcast in 128-, 256- or 512-thread blocks at P2-P4, or
+ * LDS-staged); choices[2g] = 1 for the fused element kernel, choices[2g+1] = the row-broadcast
+ * kernel's block size (0: LDS-staged) */
 int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices);
+/* serial != 0: every group on the ensemble's stream one after another (for comparison); default:
+ * each group on its own stream, forked from and joined back into the ensemble's stream per step */
+int fvdg_ensemble_set_serial(fvdg_ensemble* e, int serial);
 
 #ifdef __cplusplus
 }

@@ -114,3 +114,20 @@ def test_mixed_precision_ensemble_tracks_fp64():
         for p in range(len(SPECS)):
             a, b = out[prec][p], out["fp64"][p]
             assert np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()) < 1e-6, (prec, p)
+
+
+def test_concurrent_groups_equal_serial_groups():
+    """order groups on their own streams (forked and joined per step, also inside the graph) give
+    the same states bit for bit as the groups one after another on one stream"""
+    print("[case] ensemble concurrent vs serial groups", flush=True)
+    _, conc, _ = build()
+    _, ser, _ = build()
+    ser.set_serial(True)
+    for e in (conc, ser):
+        e.set_dt(np.full(len(SPECS), 0.01))
+        e.step(2, graph=True)
+        e.step(1, graph=False)
+    for p in range(len(SPECS)):
+        assert np.array_equal(conc.download(p), ser.download(p)), p
+    conc.close()
+    ser.close()
