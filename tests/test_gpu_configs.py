@@ -12,8 +12,11 @@ cannot follow a long run, against FP64):
   (FP64 1e-12, mixed 2e-5), and 20 SSP-RK3 steps with the kinetic energy and enstrophy series
   (every 5 steps) of the mixed paths within 1e-8 of FP64's and the final states within 1e-6
   (relative L2).
-C3 (mixed precision, jittered, P3) is covered by test_gpu_mixed.py at 16^3 x 6 and by the
-benchmark at full size; C4 (the ensemble) by test_gpu_ensemble.py."""
+* C3 at full size: the 1,296,000-tet jittered cube's FP64 (1e-12) and FP32-contraction (2e-5) residuals
+  against the oracle's (trajectories at 16^3 x 6 in test_gpu_mixed.py, 23 steps at full size in the
+  benchmark's deviation figure);
+* C4's draw: the first 12 of the 256 problems (seed 256) as one graph-replayed, autotuned ensemble,
+  one step, each problem against the oracle (1e-11)."""
 import math
 
 import numpy as np
@@ -105,3 +108,51 @@ def test_c5_tgv_p4_series():
         print(f"TGV P4 {prec}: series deviation {dev}, state {r[f'l2_rel_deviation_{prec}_vs_fp64']:.2e}", flush=True)
         assert dev["kinetic_energy"] < 1e-8 and dev["enstrophy"] < 1e-8, prec
         assert r[f"l2_rel_deviation_{prec}_vs_fp64"] < 1e-6, prec
+
+
+def test_c3_residual_full_size(solver, oracle):
+    """C3 at full size: the 60^3 x 6 = 1,296,000-tet jittered periodic cube, P3 Roe; the FP64 residual
+    within 1e-12 and the FP32-contraction residual within 2e-5 of the oracle's (all host threads)."""
+    print("[case] C3: 60^3x6 jittered P3 roe residual against the oracle", flush=True)
+    mesh = F.Mesh.box(60, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=60)
+    assert mesh.n_cells == 1296000
+    U = mesh.project(3, "vortex", VORTEX)
+    Ro = oracle.residual(mesh.export(), 3, U, flux="roe", threads=oracle.max_threads())
+    for prec, tol in (("fp64", 1e-12), ("fp32", 2e-5)):
+        solver.set_scheme(3, flux="roe", precision=prec).set_mesh(mesh)
+        solver.upload(U)
+        err = rel_err(solver.residual(), Ro)
+        print(f"C3 {prec} residual vs oracle: {err:.2e}", flush=True)
+        assert err < tol
+
+
+def test_c4_subset_matches_oracle(oracle):
+    """C4's draw (seed 256: n ~ U{6..24}, P1..P4, strength U(1,5), centre U(3,7)): the first 12
+    problems as one ensemble (graph-replayed, autotuned), one SSP-RK3 step, each problem against the
+    oracle's step of that problem alone (1e-11)."""
+    print("[case] C4: first 12 problems of the 256-problem draw against the oracle", flush=True)
+    rng = np.random.default_rng(256)
+    P = 256
+    ns, orders = rng.integers(6, 25, P), rng.integers(1, 5, P)
+    strength, centre = rng.uniform(1, 5, P), rng.uniform(3, 7, (P, 3))
+    K = 12
+    meshes = [F.Mesh.box(int(ns[p]), ub=(10, 10, 10), periodic=7) for p in range(K)]
+    params = [[1.4, strength[p], centre[p, 0], centre[p, 1], 1.0, 1.0, 10.0, 10.0] for p in range(K)]
+    U0 = [meshes[p].project(int(orders[p]), "vortex", params[p]) for p in range(K)]
+    ens = F.Ensemble(meshes, [int(o) for o in orders[:K]], flux="roe")
+    try:
+        for p in range(K):
+            ens.upload(p, U0[p])
+        lam = ens.max_wave_speeds()
+        h = np.array([3.0 * m.geometry()["h"].min() for m in meshes])
+        dt = 0.3 * h / ((2 * orders[:K] + 1) * lam)
+        ens.set_dt(dt)
+        ens.autotune(trial_steps=1)
+        ens.step(1, graph=True)
+        th = oracle.max_threads()
+        for p in range(K):
+            ref = oracle.ssprk3(meshes[p].export(), int(orders[p]), U0[p], dt[p], 1, flux="roe", threads=th)
+            err = rel_err(ens.download(p), ref)
+            assert err < 1e-11, (p, int(ns[p]), int(orders[p]), err)
+    finally:
+        ens.close()
