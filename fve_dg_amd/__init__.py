@@ -75,6 +75,8 @@ def lib():
         _sig(L, "fvdg_mesh_set_zone_boundary_type", [_vp, C.c_int32, C.c_uint8])
         _sig(L, "fvdg_solution_write", [_vp, C.c_int, _d, C.c_int, _d, C.c_int, C.c_double, C.c_char_p])
         _sig(L, "fvdg_solution_read", [_vp, C.c_int, C.c_char_p, _d])
+        _sig(L, "fvdg_checkpoint_write", [C.c_char_p, C.c_int, C.c_int64, _d, C.c_double, C.c_int64])
+        _sig(L, "fvdg_checkpoint_read", [C.c_char_p, C.POINTER(C.c_int), _i64, _d, _d, _i64])
         _sig(L, "fvdg_project_analytic", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d])
         _sig(L, "fvdg_analytic_error", [_vp, C.c_int, C.c_int, _d, C.c_int, C.c_double, _d, _d])
         _sig(L, "fvdg_device_count", [C.POINTER(C.c_int)])
@@ -162,6 +164,24 @@ def quadrature(order):
     fp, fw = np.zeros((nf.value, 2)), np.zeros(nf.value)
     _check(lib().fvdg_quadrature(order, None, _p(vp), _p(vw), None, _p(fp), _p(fw)))
     return vp, vw, fp, fw
+
+
+def checkpoint_write(path, order, U, t=0.0, step=0):
+    """binary checkpoint of a state (E, 5N) at time t, step number `step`"""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    if U.ndim != 2 or U.shape[1] != 5 * num_basis(order):
+        raise ValueError(f"state shape {U.shape} does not match order {order}")
+    _check(lib().fvdg_checkpoint_write(str(path).encode(), order, U.shape[0], _p(U), t, step))
+
+
+def checkpoint_read(path):
+    """(U, order, t, step) of a binary checkpoint"""
+    o, n, t, st = C.c_int(), C.c_int64(), C.c_double(), C.c_int64()
+    p = str(path).encode()
+    _check(lib().fvdg_checkpoint_read(p, C.byref(o), C.byref(n), None, None, None))
+    U = np.zeros((n.value, 5 * num_basis(o.value)))
+    _check(lib().fvdg_checkpoint_read(p, None, None, _p(U), C.byref(t), C.byref(st)))
+    return U, o.value, t.value, st.value
 
 
 class Mesh:

@@ -3,6 +3,7 @@
 // fields used for initial states and error norms.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -617,6 +618,78 @@ int fvdg_solution_read(const fvdg_mesh* m, int order, const char* path, double* 
                     U[e * 5 * N + 5 * k + v] = s / den[k];
                 }
     });
+    return FVDG_OK;
+}
+
+// ---------------------------------------------------------------- binary checkpoints
+// Header of 56 bytes, then the raw coefficients: magic "FVDGCKP1", u32 version, i32 order, i64 cells,
+// i64 values (cells * 5N), f64 time, i64 step, u64 FNV-1a hash of the coefficient bytes.
+namespace {
+constexpr char kCkptMagic[8] = {'F', 'V', 'D', 'G', 'C', 'K', 'P', '1'};
+struct CkptHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t order;
+    int64_t cells, values;
+    double time;
+    int64_t step;
+    uint64_t hash;
+};
+static_assert(sizeof(CkptHeader) == 56, "checkpoint header layout");
+uint64_t fnv1a(const unsigned char* p, size_t n) {
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+    return h;
+}
+}  // namespace
+
+int fvdg_checkpoint_write(const char* path, int order, int64_t n_cells, const double* U, double t, int64_t step) {
+    if (!path || !U) return fail(FVDG_ERR_ARG, "null argument");
+    if (order < 0 || order > 5 || n_cells <= 0) return fail(FVDG_ERR_ARG, "order must be 0..5 and cells positive");
+    const int64_t nv = n_cells * 5 * ((order + 1) * (order + 2) * (order + 3) / 6);
+    CkptHeader h;
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.order = order;
+    h.cells = n_cells;
+    h.values = nv;
+    h.time = t;
+    h.step = step;
+    h.hash = fnv1a(reinterpret_cast<const unsigned char*>(U), (size_t)nv * sizeof(double));
+    // written to a temporary file and renamed: a crash mid-write leaves the previous checkpoint
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+        std::ofstream out(tmp, std::ios::binary | std::ios::trunc);
+        if (!out) return fail(FVDG_ERR_ARG, "cannot open " + tmp);
+        out.write(reinterpret_cast<const char*>(&h), sizeof h);
+        out.write(reinterpret_cast<const char*>(U), (std::streamsize)(nv * sizeof(double)));
+        out.close();
+        if (!out) return fail(FVDG_ERR_ARG, "write failed: " + tmp);
+    }
+    if (std::rename(tmp.c_str(), path) != 0) return fail(FVDG_ERR_ARG, std::string("cannot rename to ") + path);
+    return FVDG_OK;
+}
+
+int fvdg_checkpoint_read(const char* path, int* order, int64_t* n_cells, double* U, double* t, int64_t* step) {
+    if (!path) return fail(FVDG_ERR_ARG, "null path");
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return fail(FVDG_ERR_ARG, std::string("cannot open ") + path);
+    CkptHeader h;
+    if (!in.read(reinterpret_cast<char*>(&h), sizeof h)) return fail(FVDG_ERR_STATE, "truncated checkpoint header");
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1)
+        return fail(FVDG_ERR_ARG, "not a version-1 fvdg checkpoint");
+    if (h.order < 0 || h.order > 5 || h.cells <= 0 ||
+        h.values != h.cells * 5 * ((h.order + 1) * (h.order + 2) * (h.order + 3) / 6))
+        return fail(FVDG_ERR_STATE, "inconsistent checkpoint header");
+    if (order) *order = h.order;
+    if (n_cells) *n_cells = h.cells;
+    if (t) *t = h.time;
+    if (step) *step = h.step;
+    if (!U) return FVDG_OK;
+    if (!in.read(reinterpret_cast<char*>(U), (std::streamsize)(h.values * sizeof(double))))
+        return fail(FVDG_ERR_STATE, "truncated checkpoint data");
+    if (fnv1a(reinterpret_cast<const unsigned char*>(U), (size_t)h.values * sizeof(double)) != h.hash)
+        return fail(FVDG_ERR_STATE, "checkpoint data does not match its checksum");
     return FVDG_OK;
 }
 

@@ -268,6 +268,16 @@ int fvdg_solution_write(const fvdg_mesh* m, int order, const double* U, int fiel
  * examples/Vortex/tools.h:72-129) */
 int fvdg_solution_read(const fvdg_mesh* m, int order, const char* path, double* U);
 
+/* binary checkpoint of a state (restart without the projection error of the text files): a 56-byte
+ * header -- "FVDGCKP1", version 1, order, cells, values = cells * 5N, time, step, FNV-1a hash of the
+ * coefficient bytes -- then the raw FP64 coefficients in the LongVector layout; written to path.tmp
+ * and renamed.  The reference restarts only from its text files (examples/Vortex/tools.h:73-129). */
+int fvdg_checkpoint_write(const char* path, int order, int64_t n_cells, const double* U, double t, int64_t step);
+/* reads one back: with U == NULL only the header fields (each pointer nullable); else also the
+ * n_cells * 5N coefficients, checked against the hash.  FVDG_ERR_ARG: not a checkpoint;
+ * FVDG_ERR_STATE: truncated, inconsistent or corrupt. */
+int fvdg_checkpoint_read(const char* path, int* order, int64_t* n_cells, double* U, double* t, int64_t* step);
+
 /* ---------------- ensembles: many independent problems of uneven size and order ----------------
  * (no counterpart in the reference, whose drivers run one problem at a time.)  Problems of one
  * order are stepped as one disjoint-union mesh with per-problem time steps; one SSP-RK3 step of

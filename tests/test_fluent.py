@@ -1,4 +1,5 @@
-"""Fluent .msh ingestion (the reference's data/case11.msh, kept as a fixture in tests/golden).
+"""Fluent .msh ingestion (the reference's data/case11.msh and data/case46.msh, kept as fixtures in
+tests/golden).
 
 Checked against an independent reading of the same file in Python: the domain volume (sum of
 the cells' volumes from their face lists by the divergence theorem) equals the sum of the
@@ -15,6 +16,7 @@ import fve_dg_amd as F
 from conftest import rel_err
 
 CASE11 = Path(__file__).resolve().parent / "golden" / "case11.msh"
+CASE46 = Path(__file__).resolve().parent / "golden" / "case46.msh"  # the reference's data/case46.msh
 
 
 def parse_fluent(path):
@@ -48,7 +50,22 @@ def test_case11_reads_and_matches_an_independent_parse():
     zones = mesh.zones()
     assert {v[1] for v in zones.values()} >= {"SYM1", "SYM2", "DRIVWALL", "WALL"}
     assert zones[9][0] == 3  # wall
-    nodes, faces = parse_fluent(CASE11)
+    check_against_parse(mesh, CASE11)
+
+
+def test_case46_reads_and_matches_an_independent_parse():
+    """data/case46.msh: 8,000 hexahedra (each split into 12 tets around its centroid), zones BOT,
+    INLET (velocity inlet, type 10), OUTLET (pressure outlet, 5), SYM, TOP"""
+    mesh = F.Mesh.read_fluent(CASE46)
+    assert mesh.sizes[1] == 96000
+    zones = mesh.zones()
+    assert {v[1] for v in zones.values()} == {"FLUID", "int_FLUID", "BOT", "INLET", "OUTLET", "SYM", "TOP"}
+    assert {v[0] for v in zones.values() if v[1] in ("INLET", "OUTLET")} == {10, 5}
+    check_against_parse(mesh, CASE46)
+
+
+def check_against_parse(mesh, path):
+    nodes, faces = parse_fluent(path)
     # domain volume: sum over cells of (1/3) sum_faces (x_f . n_f) A_f with outward normals,
     # i.e. over all faces: (1/3) x_f . (n A)_f counted for c0 with the face orientation and
     # for c1 opposite -- interior faces cancel, leaving the boundary integral
@@ -124,4 +141,29 @@ def test_fluent_mesh_residual_matches_oracle(solver, oracle, order, prec):
     solver.set_dirichlet_states(UD)
     Rg = solver.residual()
     Ro = oracle.residual(a, order, U, flux="roe", bc=mesh.boundary_types(), UD=UD.reshape(-1, 5))
+    assert rel_err(Rg, Ro) < (1e-12 if prec == "fp64" else 2e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,prec", [(1, "fp64"), (2, "fp64"), (3, "fp32")])
+def test_case46_residual_matches_oracle(solver, oracle, order, prec):
+    """case46.msh (96,000 tets from 8,000 hexahedra) with Dirichlet data on its wall zones (BOT,
+    SYM, TOP) and the inlet/outlet zones contributing nothing, as in the reference: FP64 to the
+    parity bar, FP32 contractions to 2e-5."""
+    print(f"[case] case46.msh residual P{order} {prec}", flush=True)
+    mesh = F.Mesh.read_fluent(CASE46)
+    for z, (bct, _name) in mesh.zones().items():
+        if bct == 3:
+            mesh.set_zone_boundary_type(z, F.BC_DIRICHLET)
+    a = mesh.export()
+    U = mesh.project(order, "const", [1.0, 0.2, 0.1, 0.0, 2.6])
+    U *= 1 + 1e-2 * np.random.default_rng(1).standard_normal(U.shape)
+    solver.set_scheme(order, flux="lf", precision=prec).set_mesh(mesh)
+    solver.upload(U)
+    _, xyz = solver.dirichlet_points()
+    UD = np.tile([1.0, 0.25, 0.05, 0.0, 2.7], xyz.shape[:2] + (1,))
+    solver.set_dirichlet_states(UD)
+    Rg = solver.residual()
+    Ro = oracle.residual(a, order, U, flux="lf", bc=mesh.boundary_types(), UD=UD.reshape(-1, 5),
+                         threads=oracle.max_threads())
     assert rel_err(Rg, Ro) < (1e-12 if prec == "fp64" else 2e-5)
