@@ -192,6 +192,10 @@ class Mesh:
 
     @classmethod
     def box(cls, n, lb=(0.0, 0.0, 0.0), ub=(1.0, 1.0, 1.0), split=6, periodic=0, jitter=0.0, seed=0):
+        """n[0] x n[1] x n[2] hexahedra on [lb, ub] split into tets: split 6 / 5 = this library's
+        translation-invariant patterns (periodic pairing: bit d of `periodic`), split 16 / 15 =
+        the reference's OrthHexMesh + split_hex6_scan / split_hex5_scan + rebuild_cell_topology
+        (the same tets, vertex orders and face numbering; not periodic)"""
         n = (n, n, n) if np.isscalar(n) else tuple(n)
         h = _vp()
         lb_, ub_ = np.asarray(lb, dtype=np.float64), np.asarray(ub, dtype=np.float64)

@@ -307,7 +307,10 @@ int fvdg_quadrature(int order, int* n_vol, double* vol_pts, double* vol_w, int* 
 int fvdg_mesh_create_box(fvdg_mesh** out, const double lb[3], const double ub[3], const int64_t n[3], int split,
                          int periodic_mask, double jitter, uint64_t seed) {
     if (!out || !lb || !ub || !n) return fail(FVDG_ERR_ARG, "null argument");
-    if (split != 5 && split != 6) return fail(FVDG_ERR_ARG, "split must be 5 or 6");
+    if (split != 5 && split != 6 && split != 15 && split != 16)
+        return fail(FVDG_ERR_ARG, "split must be 5, 6 (periodic-capable) or 15, 16 (the reference's split)");
+    if (split > 6 && (periodic_mask & 7))
+        return fail(FVDG_ERR_ARG, "the reference's split (15, 16) has no periodic pairing: use split 5 or 6");
     for (int d = 0; d < 3; ++d) {
         if (n[d] < 1) return fail(FVDG_ERR_ARG, "n must be >= 1 along each axis");
         if (!(ub[d] > lb[d])) return fail(FVDG_ERR_ARG, "ub must exceed lb");
@@ -318,9 +321,17 @@ int fvdg_mesh_create_box(fvdg_mesh** out, const double lb[3], const double ub[3]
     try {
         OrthHexMesh g({lb[0], lb[1], lb[2]}, {ub[0], ub[1], ub[2]}, {(uInt)n[0], (uInt)n[1], (uInt)n[2]});
         if (split == 6)
-            g.split_hex6_scan();
-        else
-            g.split_hex5_scan();
+            g.split_hex6_periodic();
+        else if (split == 5)
+            g.split_hex5_periodic();
+        else {
+            // the reference's sequence (OrthHexMesh, split_hex*_scan, rebuild_cell_topology)
+            if (split == 16)
+                g.split_hex6_scan();
+            else
+                g.split_hex5_scan();
+            g.rebuild_cell_topology();
+        }
         const std::array<bool, 3> per = {(periodic_mask & 1) != 0, (periodic_mask & 2) != 0, (periodic_mask & 4) != 0};
         if (periodic_mask & 7) {
             const uInt merged = g.make_periodic({lb[0], lb[1], lb[2]}, {ub[0], ub[1], ub[2]}, per);

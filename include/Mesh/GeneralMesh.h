@@ -4,13 +4,20 @@
 // Class and method names follow the reference's include/Mesh/GeneralMesh.h:9-25
 // (m_points / m_faces / m_cells, split_hex5_scan, split_hex6_scan,
 // rebuild_cell_topology, validate_mesh) and the cell/face structs of
-// include/Mesh/{CellType.h,FaceType.h}.  The splitting itself is this
-// library's own: every hexahedron of a structured grid is split by a fixed,
-// translation-invariant pattern (6 tets around the v0-v6 diagonal, or 5 tets
-// alternating with (i+j+k) parity), which keeps the mesh conforming and makes
-// opposite boundary faces exact translates of each other, so that periodic
-// pairing works.  Tet nodes are oriented positively as in the reference's
-// Tetrahedron::reorder (CellType.h:14-23).
+// include/Mesh/{CellType.h,FaceType.h}.
+//
+// Two splittings:
+//   * split_hex5_scan / split_hex6_scan reproduce the reference's (HexSplitScan.h):
+//     followed by rebuild_cell_topology they give the reference's tetrahedra,
+//     vertex orders and face numbering for the hexahedra of an OrthHexMesh.
+//   * split_hex5_periodic / split_hex6_periodic are this library's own: every
+//     hexahedron is split by a fixed, translation-invariant pattern (6 tets around
+//     the v0-v6 diagonal, or 5 tets alternating with (i+j+k) parity), which keeps
+//     opposite boundary faces exact translates of each other, so that periodic
+//     pairing (make_periodic) works.  The reference's diagonals depend on visiting
+//     order and rounding and do not pair up across opposite planes.
+// Tet nodes are oriented positively as in the reference's Tetrahedron::reorder
+// (CellType.h:14-23).
 //
 // Additions with no counterpart in the reference (its BoundaryType,
 // include/Mesh/ComputingMesh.h:71-86, has no periodic value):
@@ -21,7 +28,9 @@
 #include <map>
 #include <random>
 #include <unordered_map>
+#include <unordered_set>
 #include <variant>
+#include "Mesh/HexSplitScan.h"
 #include "base/Type.h"
 
 //---------------- faces ----------------
@@ -35,13 +44,16 @@ struct TriangleFace {
 struct QuadFace {
     vector4u nodes;
     vector2u neighbor_cells = {uInt(-1), uInt(-1)};
+    vector2u diagonal_nodes = {uInt(-1), uInt(-1)};  // cutting diagonal, once the split decides it
 };
 using GeneralFace = std::variant<TriangleFace, QuadFace>;
 
 //---------------- cells ----------------
 struct Tetrahedron {
     vector4u nodes;
-    vector4u faces = {uInt(-1), uInt(-1), uInt(-1), uInt(-1)};  // faces[l] is opposite nodes[l]
+    // the cell's faces: faces[l] opposite nodes[l] when this library builds the faces, in
+    // ascending face order after rebuild_cell_topology on existing faces (as the reference)
+    vector4u faces = {uInt(-1), uInt(-1), uInt(-1), uInt(-1)};
     // positive orientation: swap nodes 1 and 2 if (p1-p0, p2-p0, p3-p0) is left-handed
     void reorder(const std::vector<vector3f>& pts) {
         const auto v1 = pts[nodes[1]] - pts[nodes[0]];
@@ -53,7 +65,9 @@ struct Tetrahedron {
 // Hexahedron corners: v0=(0,0,0) v1=(1,0,0) v2=(1,1,0) v3=(0,1,0), v4..v7 the same at z=1.
 struct Hexahedron {
     vector8u nodes;
-    uInt parity = 0;  // (i+j+k) mod 2 of a structured hex; selects the 5-tet pattern
+    // quadrilateral faces in ascending face order; opposite pairs (0,1), (2,3), (4,5)
+    std::array<uInt, 6> faces = {uInt(-1), uInt(-1), uInt(-1), uInt(-1), uInt(-1), uInt(-1)};
+    uInt parity = 0;  // (i+j+k) mod 2 of a structured hex; selects the periodic 5-tet pattern
 };
 using GeneralCell = std::variant<Hexahedron, Tetrahedron>;
 
@@ -63,16 +77,27 @@ public:
     std::vector<GeneralFace> m_faces;
     std::vector<GeneralCell> m_cells;
 
-    void split_hex5_scan() { split_hex(true); }
-    void split_hex6_scan() { split_hex(false); }
+    // the reference's splittings (GeneralMesh.h:24-25); call rebuild_cell_topology after
+    void split_hex5_scan() { split_scan(true); }
+    void split_hex6_scan() { split_scan(false); }
+    // this library's translation-invariant splittings (faces rebuilt, tets oriented)
+    void split_hex5_periodic() { split_hex(true); }
+    void split_hex6_periodic() { split_hex(false); }
 
-    // Rebuild the triangle faces of a pure tet mesh from the cells: one face per
-    // distinct node triple, numbered in order of first appearance over
-    // (cell, local face); face nodes in the first cell's local order; side-0 cell
-    // is the first cell seen.  A no-op while hexahedra remain.
+    // Pure tet mesh.  With faces present (after split_hex*_scan), as the reference's
+    // rebuild_cell_topology (GeneralMesh.cpp:4-52): each cell's faces in ascending
+    // order, its nodes in the iteration order of a std::unordered_set<uint64_t> filled
+    // with those faces' nodes in that order, then oriented positively.  Without faces,
+    // builds them: one face per distinct node triple, numbered in order of first
+    // appearance over (cell, local face); face nodes in the first cell's local order;
+    // side-0 cell is the first cell seen.  A no-op while hexahedra remain.
     void rebuild_cell_topology() {
         for (const auto& c : m_cells)
             if (!std::holds_alternative<Tetrahedron>(c)) return;
+        if (!m_faces.empty()) {
+            reorder_from_faces();
+            return;
+        }
         std::unordered_map<Key3, uInt, Key3Hash> index;
         index.reserve(m_cells.size() * 3);
         std::vector<GeneralFace> faces;
@@ -211,6 +236,63 @@ public:
     }
 
 private:
+    void reorder_from_faces() {
+        std::vector<std::vector<uInt>> cell_faces(m_cells.size());
+        for (uInt f = 0; f < m_faces.size(); ++f) {
+            const vector2u nc = std::visit([](auto&& x) { return x.neighbor_cells; }, m_faces[f]);
+            for (uInt c : nc)
+                if (c != uInt(-1)) {
+                    if (c >= m_cells.size()) throw std::runtime_error("face references a missing cell");
+                    cell_faces[c].push_back(f);
+                }
+        }
+        for (uInt c = 0; c < m_cells.size(); ++c) {
+            auto& t = std::get<Tetrahedron>(m_cells[c]);
+            if (cell_faces[c].size() != 4) throw std::runtime_error("Cell face count mismatch");
+            std::unordered_set<uInt> nodes;
+            for (uInt f : cell_faces[c]) {
+                const auto* tf = std::get_if<TriangleFace>(&m_faces[f]);
+                if (!tf) throw std::runtime_error("tetrahedron with a quadrilateral face");
+                for (uInt v : tf->nodes) nodes.insert(v);
+            }
+            if (nodes.size() != 4) throw std::runtime_error("Cell node count mismatch");
+            std::copy(nodes.begin(), nodes.end(), t.nodes.begin());
+            std::copy(cell_faces[c].begin(), cell_faces[c].end(), t.faces.begin());
+            t.reorder(m_points);
+        }
+    }
+
+    // the reference's splitting of the hexahedra of an OrthHexMesh (HexSplitScan.h)
+    void split_scan(bool five) {
+        std::vector<fvdg_scan::Hex> hexes;
+        hexes.reserve(m_cells.size());
+        for (const auto& c : m_cells) {
+            const auto* h = std::get_if<Hexahedron>(&c);
+            if (!h || h->faces[0] == uInt(-1))
+                throw std::runtime_error("split_hex*_scan: needs the hexahedra and faces of an OrthHexMesh");
+            hexes.push_back({h->nodes, h->faces});
+        }
+        std::vector<fvdg_scan::Quad> quads(m_faces.size());
+        for (uInt f = 0; f < m_faces.size(); ++f) {
+            const auto* q = std::get_if<QuadFace>(&m_faces[f]);
+            if (!q) throw std::runtime_error("split_hex*_scan: hexahedron faces must be quadrilaterals");
+            quads[f].nodes = q->nodes;
+            quads[f].cells = q->neighbor_cells;
+            quads[f].diag = q->diagonal_nodes;
+        }
+        std::vector<vector4u> tets;
+        std::vector<fvdg_scan::Face> tris;
+        fvdg_scan::Splitter(m_points).run(hexes, quads, five, tets, tris);
+        m_cells.assign(tets.size(), Tetrahedron{});
+        for (uInt c = 0; c < tets.size(); ++c) std::get<Tetrahedron>(m_cells[c]).nodes = tets[c];
+        m_faces.assign(tris.size(), TriangleFace{});
+        for (uInt f = 0; f < tris.size(); ++f) {
+            auto& tf = std::get<TriangleFace>(m_faces[f]);
+            tf.nodes = {tris[f].nodes[0], tris[f].nodes[1], tris[f].nodes[2]};
+            tf.neighbor_cells = tris[f].cells;
+        }
+    }
+
     static Scalar plane_tolerance(const vector3f& lb, const vector3f& ub) {
         Scalar L = 0;
         for (int d = 0; d < 3; ++d) L = std::max(L, ub[d] - lb[d]);

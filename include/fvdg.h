@@ -81,7 +81,11 @@ int fvdg_quadrature(int order, int* n_vol, double* vol_pts, double* vol_w, int* 
                     double* face_w);
 
 /* ------------------------------ host mesh ------------------------------ */
-/* Box [lb,ub] of n[0]*n[1]*n[2] hexahedra split into 6 (split=6) or 5 (split=5) tets.
+/* Box [lb,ub] of n[0]*n[1]*n[2] hexahedra split into tets.  split = 6 / 5: this library's
+ * translation-invariant 6- / 5-tet patterns (periodic pairing possible); split = 16 / 15: the
+ * reference's OrthHexMesh + split_hex6_scan / split_hex5_scan + rebuild_cell_topology
+ * (include/Mesh/OrthHexMesh.h:3-106, src/Mesh/GeneralMesh.cpp:4-300), the same tets, vertex
+ * orders and face numbering (no periodic pairing).
  * periodic_mask bit d pairs the faces on x_d = lb_d and x_d = ub_d.  jitter moves every
  * vertex class by U(-jitter*h, jitter*h) per coordinate (seeded, periodic images alike). */
 int fvdg_mesh_create_box(fvdg_mesh** out, const double lb[3], const double ub[3], const int64_t n[3], int split,

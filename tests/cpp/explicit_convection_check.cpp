@@ -1,5 +1,6 @@
 // A reference-style caller of the C++ drop-in headers: it builds a mesh with
-// OrthHexMesh + split_hex6_scan + ComputingMesh, tags boundaries like the vortex
+// OrthHexMesh + split_hex6_scan + rebuild_cell_topology + ComputingMesh (the reference's
+// mesh; periodic cases: split_hex6_periodic + make_periodic), tags boundaries like the vortex
 // example (examples/Vortex/tools.h:22-31), defines the exact-solution hooks of
 // base/exact.h, projects the vortex with DGBasisEvaluator::func2coef, and calls
 // ExplicitConvection<P>::eval / eval_cells / eval_internals / eval_boundarys, the
@@ -101,6 +102,7 @@ template <uInt P>
 void boundary_case(Writer& w, int flux) {
     OrthHexMesh g({0, 0, 0}, {10, 10, 2.5}, {4, 4, 2});
     g.split_hex6_scan();
+    g.rebuild_cell_topology();
     ComputingMesh cm(g);
     for (size_t f = 0; f < cm.m_faces.size(); ++f)
         if (cm.m_faces[f].is_boundary())
@@ -138,7 +140,7 @@ void boundary_case(Writer& w, int flux) {
 template <uInt P>
 void periodic_case(Writer& w, int flux) {
     OrthHexMesh g({0, 0, 0}, {10, 10, 10}, {3, 3, 3});
-    g.split_hex6_scan();
+    g.split_hex6_periodic();
     g.make_periodic({0, 0, 0}, {10, 10, 10}, {true, true, true});
     g.jitter_points(0.1, 7, {0, 0, 0}, {10, 10, 10}, {10.0 / 3, 10.0 / 3, 10.0 / 3}, {true, true, true});
     ComputingMesh cm(g);
@@ -167,7 +169,7 @@ template <uInt P>
 void limiter_case(Writer& w) {
     constexpr uInt N = DGBasisEvaluator<P>::NumBasis;
     OrthHexMesh g({0, 0, 0}, {10, 10, 10}, {3, 3, 3});
-    g.split_hex6_scan();
+    g.split_hex6_periodic();
     g.make_periodic({0, 0, 0}, {10, 10, 10}, {true, true, true});
     g.jitter_points(0.1, 9, {0, 0, 0}, {10, 10, 10}, {10.0 / 3, 10.0 / 3, 10.0 / 3}, {true, true, true});
     ComputingMesh cm(g);
@@ -200,7 +202,7 @@ template <uInt P>
 void implicit_case(Writer& w) {
     constexpr uInt NB = 5 * DGBasisEvaluator<P>::NumBasis;
     OrthHexMesh g({0, 0, 0}, {10, 10, 10}, {3, 3, 3});
-    g.split_hex6_scan();
+    g.split_hex6_periodic();
     g.make_periodic({0, 0, 0}, {10, 10, 10}, {true, true, true});
     g.jitter_points(0.1, 11, {0, 0, 0}, {10, 10, 10}, {10.0 / 3, 10.0 / 3, 10.0 / 3}, {true, true, true});
     ComputingMesh cm(g);

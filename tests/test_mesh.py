@@ -144,6 +144,41 @@ def test_reference_mesh_fixture_matches_reference_build(ref):
         assert np.array_equal(m[k], d[f"box6_{k}"])
 
 
+def _export4(m):
+    a = m.export()
+    return {k: a[k] for k in ("points", "cells", "face_nodes", "face_cells")}
+
+
+def test_reference_split_reproduces_the_fixture():
+    """split 16 / 15 (OrthHexMesh + split_hex6_scan / split_hex5_scan + rebuild_cell_topology)
+    gives the reference's mesh exactly: the committed fixture, built by the reference's code."""
+    d = np.load(GOLDEN)
+    for name, n, split in (("box6", (3, 3, 2), 16), ("box5", (2, 3, 2), 15)):
+        m = _export4(F.Mesh.box(n, ub=(10, 10, 2.5), split=split))
+        for k in ("points", "cells", "face_nodes", "face_cells"):
+            assert np.array_equal(m[k], d[f"{name}_{k}"]), (name, k)
+
+
+@pytest.mark.parametrize("n,ub", [((1, 1, 1), (1, 1, 1)), ((2, 2, 2), (1, 1, 1)), ((4, 4, 1), (10, 10, 2.5)),
+                                  ((5, 3, 2), (2, 1, 3)), ((8, 8, 8), (10, 10, 10)), ((16, 16, 1), (10, 10, 0.625)),
+                                  ((7, 6, 5), (1, 1.3, 0.7))])
+@pytest.mark.parametrize("split", [6, 5])
+def test_reference_split_matches_reference_build(ref, n, ub, split):
+    """bit-identical to the reference's own mesh code (oracle/_ref, built from its sources): the
+    same points, tets with the same vertex orders, faces with the same numbering, vertex orders
+    and cells, on cubes, slabs and uneven boxes (square and non-square faces: the diagonal
+    choice of a square face is a floating-point tie)"""
+    r = ref.box_mesh(n, (0, 0, 0), ub, split)
+    m = _export4(F.Mesh.box(n, ub=ub, split=10 + split))
+    for k in ("points", "cells", "face_nodes", "face_cells"):
+        assert np.array_equal(m[k], r[k]), k
+
+
+def test_reference_split_rejects_periodic_pairing():
+    with pytest.raises(F.FvdgError):
+        F.Mesh.box(4, split=16, periodic=1)
+
+
 def test_library_box_is_a_valid_split_of_the_same_hexes(ref):
     """The library's 6-split and the reference's 6-split tile the same box with the same
     number of tets and faces (different diagonal choices are allowed)."""
