@@ -287,7 +287,7 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                 }
                 double* p = sU + col * STR + 5 * j;
                 if (FUSE) {
-                    const double inv_m = e < a.E ? 1.0 / (det * a.mhat[j]) : 0.0;
+                    const double inv_m = e < a.E ? rcp_pos(det * a.mhat[j]) : 0.0;
 #pragma unroll
                     for (int v = 0; v < 5; ++v) p[v] = p[v] + dte * (-rr[v] * inv_m);
                 } else {

@@ -51,8 +51,32 @@ __device__ __forceinline__ int64_t cidx(int64_t e, int kv, int NV) { return e * 
 // (rho = max(U0, 1e-16), u = U1/rho, ..., p = (gamma-1)(rho e - rho |u|^2 / 2),
 // c = sqrt(gamma max(p / max(U0,1e-16), 1e-16))), evaluated once per state with
 // one reciprocal 1/rho instead of the reference's repeated divisions (each
-// quotient is then within one rounding of the reference's).
+// quotient is then within about one rounding of the reference's).
 __device__ __forceinline__ double pos_eps(double x) { return fmax(x, 1e-16); }
+
+// Reciprocal and square root of a positive normal FP64 value (every caller clamps its argument
+// to >= 1e-16): the hardware approximations v_rcp_f64 / v_rsq_f64 (about 2^-23 relative) refined
+// by two Newton-Raphson steps / one Goldschmidt step and one residual correction, to within about
+// one ulp.  The correctly rounded IEEE sequences add argument scaling and fix-ups and cost about
+// twice the instructions; the face-flux kernel is bound by VALU issue.  A NaN argument gives NaN.
+__device__ __forceinline__ double rcp_pos(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double sqrt_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = 0.5 * y;
+    const double r = fma(-h, s, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    const double d = fma(-s, s, x);
+    return fma(d, h, s);
+}
+// x >= 0 (a sum of squares): 0 stays 0, NaN stays NaN
+__device__ __forceinline__ double sqrt_nonneg(double x) { return x > 0.0 ? sqrt_pos(x) : x; }
 
 // maximum that keeps a NaN operand (fmax drops it)
 __device__ __forceinline__ double max_keep_nan(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
@@ -70,7 +94,7 @@ struct Prim {
 __device__ __forceinline__ Prim euler_prim(const double U[5], double gamma) {
     Prim q;
     q.rho = pos_eps(U[0]);
-    const double ri = 1.0 / q.rho;
+    const double ri = rcp_pos(q.rho);
     q.ri = ri;
     q.u = U[1] * ri;
     q.v = U[2] * ri;
@@ -109,8 +133,8 @@ __device__ __forceinline__ void euler_flux_n(const Prim& q, const double n[3], d
 
 // |u| + c; c uses p / max(U0, 1e-16) = p / rho
 __device__ __forceinline__ double euler_wave_speed_prim(const Prim& q, double gamma) {
-    const double c = sqrt(gamma * pos_eps(q.p * q.ri));
-    return sqrt(q.u * q.u + q.v * q.v + q.w * q.w) + c;
+    const double c = sqrt_pos(gamma * pos_eps(q.p * q.ri));
+    return sqrt_nonneg(q.u * q.u + q.v * q.v + q.w * q.w) + c;
 }
 
 __device__ __forceinline__ double euler_wave_speed(const double U[5], double gamma) {
@@ -152,17 +176,19 @@ __device__ __forceinline__ void flux_roe(const double UL[5], const double UR[5],
     euler_flux_n(R, n, fr);
     const double uL[3] = {L.u, L.v, L.w}, uR[3] = {R.u, R.v, R.w};
     const double HL = (UL[4] + L.p) * L.ri, HR = (UR[4] + R.p) * R.ri;
-    const double sL = sqrt(L.rho), sR = sqrt(R.rho), inv = 1.0 / (sL + sR);
+    // Roe averages with the weight ratio k = sqrt(rho_R / rho_L): (x_L + k x_R) / (1 + k) equals
+    // (sqrt(rho_L) x_L + sqrt(rho_R) x_R) / (sqrt(rho_L) + sqrt(rho_R)), and sqrt(rho_L rho_R) = k rho_L
+    const double k = sqrt_pos(R.rho * L.ri), inv = rcp_pos(1.0 + k);
     double ut[3], q2 = 0, qn = 0;
     for (int d = 0; d < 3; ++d) {
-        ut[d] = (sL * uL[d] + sR * uR[d]) * inv;
+        ut[d] = (uL[d] + k * uR[d]) * inv;
         q2 += ut[d] * ut[d];
         qn += ut[d] * n[d];
     }
-    const double Ht = (sL * HL + sR * HR) * inv;
+    const double Ht = (HL + k * HR) * inv;
     const double a2 = fmax((gamma - 1) * (Ht - 0.5 * q2), 1e-16);
-    const double a = sqrt(a2), ia2 = 1.0 / a2;
-    const double rt = sL * sR;
+    const double a = sqrt_pos(a2), ia2 = rcp_pos(a2);
+    const double rt = k * L.rho;
     const double dr = R.rho - L.rho, dp = R.p - L.p;
     double du[3], dqn = 0;
     for (int d = 0; d < 3; ++d) {
