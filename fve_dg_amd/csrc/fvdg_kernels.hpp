@@ -203,10 +203,10 @@ __device__ __forceinline__ void flux_roe(const double UL[5], const double UR[5],
     const double a1 = (dp - rt * a * dqn) * (0.5 * ia2);
     const double a5 = (dp + rt * a * dqn) * (0.5 * ia2);
     const double a_2 = dr - dp * ia2;
-    const double delta = 0.1 * a;
-    auto fix = [delta](double lam) {
+    const double delta = 0.1 * a, inv_2delta = 5.0 * (a * ia2);  // 1 / (2 delta) = 5 / a = 5 a / a^2
+    auto fix = [delta, inv_2delta](double lam) {
         const double al = fabs(lam);
-        return al < delta ? (lam * lam + delta * delta) / (2 * delta) : al;
+        return al < delta ? (lam * lam + delta * delta) * inv_2delta : al;
     };
     const double l1 = fix(qn - a), l2 = fabs(qn), l5 = fix(qn + a);
     const double w1 = l1 * a1, w2 = l2 * a_2, w5 = l5 * a5, ws = l2 * rt;
