@@ -87,6 +87,18 @@ struct fvdg_ctx {
     DevBuf<int32_t> tPerm;                // face-rule point permutations of the 3 rotations
     DevBuf<double> tPhiB;                 // canonical tables as MFMA B operands (face test)
     DevBuf<float> tPhiB32;
+    // face tiles (k_face_tile, P2-P4): faces grouped by (side-0, side-1 or boundary) vertex
+    // combination in tiles of 16, the per-combination trace and test operand tables, and the
+    // per (face, side) test contributions
+    int face_tile = 1;                    // 0 off, 1 mixed precision (FP32), 2 also FP64 (FVDG_FACE_TILE)
+    int64_t n_face_tiles = 0;
+    DevBuf<int32_t> trec, tclass;         // n_face_tiles*16 column records (int4: face or -1, cells, kind),
+                                          // n_face_tiles classes
+    DevBuf<double> tAt, tAe;              // FP64 operand layouts
+    DevBuf<float> tAt32, tAe32;           // FP32 operand layouts
+    DevBuf<float> tICc32;                 // [24][NP] sum_g w_g Phi_cb[g][j] (FP32 side-1 reference term)
+    DevBuf<double> Cf;                    // FP64 contributions [F][2][5][NP]
+    DevBuf<float> Cf32;                   // FP32 contributions
     int n_cu = 256;
     // device state: U (state), Ua/Ub (stage ping-pong), R (residual)
     DevBuf<double> U, Ua, Ub, R;

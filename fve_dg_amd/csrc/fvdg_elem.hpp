@@ -56,6 +56,8 @@ struct ElemArgs {
     double a0, b0, dt;
     const double* edt;
     int* bad;
+    const T* Cf;          // CF: per (face, side) test contributions [F][2][5][NP] of k_face_tile
+    double phi0;          // CF, split: the constant value of phi_0 (MF in Fref: the faces' flux integrals)
 };
 
 template <int ORDER, typename T>
@@ -68,7 +70,7 @@ struct ElemShape {
     static constexpr bool SUPPORTED = N <= 35;
 };
 
-template <int ORDER, typename T, bool FUSE>
+template <int ORDER, typename T, bool FUSE, bool CF = false>
 __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_mfma(ElemArgs<T> a) {
     using S = VolShape<ORDER>;
     using M = Mfma<T>;
@@ -192,8 +194,64 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
     }
     int slot_c[4] = {-1, -1, -1, -1};  // split: each slot's table and face (-f-1: side 1), for step 5
     int64_t slot_f[4] = {0, 0, 0, 0};
+    // CF, split: the cell-mean row (j = 0, entry 0 of lane group 0) from the faces' FP64 flux
+    // integrals MF (sides with opposite signs: the cell means stay conserved in FP64)
+    double mf[5] = {0, 0, 0, 0, 0};
+    if constexpr (CF) {
+        // the four (face, side) contribution blocks of k_face_tile: lane (element col, grp) adds the
+        // entries of its accumulator rows (FP32: 4 contiguous test functions per load).  Loads are
+        // unconditional (face 0 for unused slots, rows clamped into the block) and their values
+        // selected: a divergent branch would wait for each load.  Two slots in flight at a time.
+        constexpr int NP = (N + 3) / 4 * 4;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s == 2) asm volatile("" ::: "memory");
+            const int32_t slot = s == 0 ? cf4.x : s == 1 ? cf4.y : s == 2 ? cf4.z : cf4.w;
+            const int meta = s == 0 ? cm4.x : s == 1 ? cm4.y : s == 2 ? cm4.z : cm4.w;
+            const bool bnd = (meta & 0x80) != 0;
+            const bool use = e < a.E && ((bnd && do_bnd) || (!bnd && do_int));
+            const int32_t sl = use ? slot : 0;
+            const T* p = a.Cf + (int64_t)sl * 5 * NP;
+            if constexpr (SPLIT) {
+                const double* q = a.Fref + (int64_t)(sl >> 1) * 5;
+                const double sg = !use ? 0.0 : (sl & 1) ? -1.0 : 1.0;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) mf[v] = fma(sg, q[v], mf[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt) {
+                    if constexpr (SPLIT) {
+                        const int j0 = 16 * jt + 4 * grp;
+                        if (16 * jt < NP) {  // rows j0 >= NP: an in-block load, discarded
+                            const float4 x = *reinterpret_cast<const float4*>(p + v * NP + min(j0, NP - 4));
+                            const bool in = use && j0 < NP;
+                            racc[v][jt][0] += (in && j0 > 0) ? x.x : 0.0f;  // j = 0: from mf
+                            racc[v][jt][1] += in ? x.y : 0.0f;
+                            racc[v][jt][2] += in ? x.z : 0.0f;
+                            racc[v][jt][3] += in ? x.w : 0.0f;
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int j = 16 * jt + M::row(r, grp);
+                            const T y = p[v * NP + min(j, N - 1)];
+                            racc[v][jt][r] += (use && j < N) ? y : (T)0;
+                        }
+                    }
+                }
+            // the sums are formed here, not sunk into the epilogue (which would keep every
+            // loaded block alive until then)
+            if (s & 1)
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+#pragma unroll
+                    for (int jt = 0; jt < JT; ++jt) asm volatile("" : "+v"(racc[v][jt]));
+        }
+    }
 #pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < (CF ? 0 : 4); ++s) {
         int cl = -1, m = 0;
         int64_t f = 0;
         T sign = 1;
@@ -251,7 +309,7 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
     //    U + dt (-R / M) replaces U in the LDS tile (the same lane reads and writes the entry)
     double fref[4][5];
     int fc[4] = {-1, -1, -1, -1};
-    if (SPLIT) {
+    if (SPLIT && !CF) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             fc[s] = slot_c[s];
@@ -271,7 +329,10 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                 double rr[5];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) rr[v] = (double)racc[v][jt][r];
-                if (SPLIT) {
+                if (CF && SPLIT && j == 0)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) rr[v] += a.phi0 * mf[v];
+                if (SPLIT && !CF) {
                     if (do_cells) {
                         const double g0 = a.Gs[3 * j], g1 = a.Gs[3 * j + 1], g2 = a.Gs[3 * j + 2];
 #pragma unroll

@@ -32,6 +32,7 @@
 #include "fvdg_diagnostics.hpp"
 #include "fvdg_bf16x3.hpp"
 #include "fvdg_elem.hpp"
+#include "fvdg_facetile.hpp"
 
 using namespace fvdg;
 
@@ -146,6 +147,55 @@ int upload_bf16x3_tables(fvdg_ctx* c) {
     }
 }
 
+// Per vertex combination cb (24) the face-tile operand tables of k_face_tile in the lane layouts of
+// TileLayout<T> (zero outside the table):
+//   At[cb][mb][ks][lane] = Phi_cb[row_point(mb, lane % 16)][4 ks + lane / 16]       (traces)
+//   Ae[cb][jt][ks][lane] = Phi_cb[point(ks, lane / 16)][16 jt + lane % 16]         (test)
+// Orders whose face rule has not 24 points get empty tables (the face-tile path is not used).
+template <typename T>
+int upload_face_tile_tables(fvdg_ctx* c, DevBuf<T>& At, DevBuf<T>& Ae, DevBuf<T>* ICc) {
+    const SchemeTables& S = c->tab;
+    if (S.Qf != 24 || S.N < 10 || S.N > 35) {
+        At.release();
+        Ae.release();
+        if (ICc) ICc->release();
+        return FVDG_OK;
+    }
+    using LY = TileLayout<T>;
+    const int N = S.N, KS = (N + 3) / 4, JT = (N + 15) / 16;
+    auto phi = [&](int cb, int g, int k) { return S.Phi[((size_t)cb * S.Qf + g) * N + k]; };
+    std::vector<T> at((size_t)24 * 2 * KS * 64, (T)0), ae((size_t)24 * JT * 6 * 64, (T)0);
+    for (int cb = 0; cb < 24; ++cb) {
+        for (int mb = 0; mb < 2; ++mb)
+            for (int ks = 0; ks < KS; ++ks)
+                for (int ln = 0; ln < 64; ++ln) {
+                    const int g = LY::row_point(mb, ln % 16), k = 4 * ks + ln / 16;
+                    if (g >= 0 && k < N) at[(((size_t)cb * 2 + mb) * KS + ks) * 64 + ln] = (T)phi(cb, g, k);
+                }
+        for (int jt = 0; jt < JT; ++jt)
+            for (int ks = 0; ks < 6; ++ks)
+                for (int ln = 0; ln < 64; ++ln) {
+                    const int g = LY::point(ks, ln / 16), j = 16 * jt + ln % 16;
+                    if (j < N) ae[(((size_t)cb * JT + jt) * 6 + ks) * 64 + ln] = (T)phi(cb, g, j);
+                }
+    }
+    HIP_TRY(At.upload(at, c->stream));
+    HIP_TRY(Ae.upload(ae, c->stream));
+    if (ICc) {
+        // ICc[cb][j] = sum_g w_g Phi_cb[g][j], rows padded to NP
+        const int NP = (N + 3) / 4 * 4;
+        std::vector<T> ic((size_t)24 * NP, (T)0);
+        for (int cb = 0; cb < 24; ++cb)
+            for (int j = 0; j < N; ++j) {
+                double sum = 0.0;
+                for (int g = 0; g < S.Qf; ++g) sum += S.fq_w[g] * phi(cb, g, j);
+                ic[(size_t)cb * NP + j] = (T)sum;
+            }
+        HIP_TRY(ICc->upload(ic, c->stream));
+    }
+    return FVDG_OK;
+}
+
 int upload_tables(fvdg_ctx* c) {
     const SchemeTables& T = c->tab;
     HIP_TRY(c->tV.upload(T.V, c->stream));
@@ -209,9 +259,25 @@ int upload_tables(fvdg_ctx* c) {
             for (int g = 0; g < T.Qf; ++g) IC[cc * T.N + j] += T.fq_w[g] * PC[((size_t)cc * T.Qf + g) * T.N + j];
     HIP_TRY(c->tGs.upload(Gs, c->stream));
     HIP_TRY(c->tIC.upload(IC, c->stream));
+    {
+        int rc = upload_face_tile_tables<double>(c, c->tAt, c->tAe, nullptr);
+        if (rc) return rc;
+        rc = upload_face_tile_tables<float>(c, c->tAt32, c->tAe32, &c->tICc32);
+        if (rc) return rc;
+    }
     HIP_TRY(c->tCanon.upload(canon, c->stream));
     HIP_TRY(c->tPerm.upload(perm, c->stream));
     return upload_bf16x3_tables(c);
+}
+
+// the face-tile path (k_face_tile + k_elem_mfma with contributions): P2-P4 (24-point face rule),
+// FP32 contractions by default, FP64 too with FVDG_FACE_TILE=all
+bool use_face_tile(const fvdg_ctx* c) {
+    if (c->face_tile == 0 || c->simple || c->owner_faces || c->lds_face_test || c->lds_face_flux || !c->elem_fused)
+        return false;
+    if (c->order < 2 || c->order > 4 || c->tab.Qf != 24 || c->n_face_tiles == 0) return false;
+    if (c->precision == FVDG_PREC_FP32) return true;
+    return c->precision == FVDG_PREC_FP64 && c->face_tile == 2;
 }
 
 // Scratch buffers of the contraction precision (face fluxes, and for the mixed path the
@@ -224,6 +290,15 @@ int precision_buffers(fvdg_ctx* c) {
     } else {
         if (c->Fh32.n != nfh) HIP_TRY(c->Fh32.alloc(nfh));
         if (c->Fref.n != (size_t)c->F * 5) HIP_TRY(c->Fref.alloc((size_t)c->F * 5));
+    }
+    // face-tile contributions, allocated when the path serves this precision
+    const size_t ncf = (size_t)c->F * 2 * 5 * ((c->tab.N + 3) / 4 * 4);
+    if (use_face_tile(c)) {
+        if (c->precision == FVDG_PREC_FP64) {
+            if (c->Cf.n != ncf) HIP_TRY(c->Cf.alloc(ncf));
+        } else if (c->Cf32.n != ncf) {
+            HIP_TRY(c->Cf32.alloc(ncf));
+        }
     }
     return FVDG_OK;
 }
@@ -540,12 +615,45 @@ void faces_fast(fvdg_ctx* c, const double* U, int parts, bool have_vol, const St
 
 // The fused element path: the face-flux pass, then k_elem_mfma (volume + face test + stage
 // update, or the residual of the selected parts into c->R).  ms_flux / ms_elem: event times.
+// the face-tile pass: every face's traces, Riemann flux and test contributions of both sides
+template <int P, typename T>
+void face_tile_pass(fvdg_ctx* c, const double* U, const T* At, const T* Ae, T* Cf, double* ms_flux) {
+    if constexpr (TileShape<P>::SUPPORTED) {
+        const T* ICc = nullptr;  // FP32 only (the FP64 tiles take no reference flux)
+        if constexpr (!std::is_same<T, double>::value) ICc = c->tICc32.p;
+        FaceTileArgs<T> f{c->n_face_tiles, c->gamma, c->flux, U, reinterpret_cast<const int4*>(c->trec.p),
+                          c->tclass.p, c->fgeo.p, c->UD.p, c->tWf.p, At, Ae, ICc, Cf, c->Fref.p};
+        Span s1(ms_flux, c->stream);
+        hipLaunchKernelGGL((k_face_tile<P, T>), dim3((unsigned)c->n_face_tiles), dim3(64), 0, c->stream, f);
+        s1.stop();
+    }
+}
+
 template <int P, typename T>
 void elem_path_t(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up, const T* PhiC, T* Fh,
                  const T* VA, const T* GA, const T* PhiB, double* ms_flux, double* ms_elem) {
     if constexpr (ElemShape<P, T>::SUPPORTED) {
-        if (parts & (FVDG_PART_INTERNALS | FVDG_PART_BOUNDARYS)) flux_pass<P, T>(c, U, PhiC, Fh, ms_flux);
+        // FP32 face tiles take every difference from the cell's own mean flux and leave out the
+        // terms that cancel between the volume and the faces: the whole residual only (single
+        // parts take the per-face reference path)
+        const bool tiles = TileShape<P>::SUPPORTED && use_face_tile(c) &&
+                           (std::is_same<T, double>::value || parts == FVDG_PART_ALL);
+        T* Cf = nullptr;
+        if (tiles) {
+            const T* At;
+            const T* Ae;
+            if constexpr (std::is_same<T, double>::value) {
+                At = c->tAt.p, Ae = c->tAe.p, Cf = c->Cf.p;
+            } else {
+                At = c->tAt32.p, Ae = c->tAe32.p, Cf = c->Cf32.p;
+            }
+            if (parts & (FVDG_PART_INTERNALS | FVDG_PART_BOUNDARYS)) face_tile_pass<P, T>(c, U, At, Ae, Cf, ms_flux);
+        } else if (parts & (FVDG_PART_INTERNALS | FVDG_PART_BOUNDARYS)) {
+            flux_pass<P, T>(c, U, PhiC, Fh, ms_flux);
+        }
         ElemArgs<T> a{};
+        a.Cf = Cf;
+        a.phi0 = c->tab.V[0];  // phi_0 is constant: its value at any point
         a.E = c->E;
         a.gamma = c->gamma;
         a.parts = parts;
@@ -575,10 +683,17 @@ void elem_path_t(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up,
             a.dt = up->dt;
         }
         Span se(ms_elem, c->stream);
-        if (up)
-            hipLaunchKernelGGL((k_elem_mfma<P, T, true>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, a);
-        else
-            hipLaunchKernelGGL((k_elem_mfma<P, T, false>), dim3(nblocks(c->E, 16)), dim3(64), 0, c->stream, a);
+        const dim3 ge(nblocks(c->E, 16));
+        if (tiles) {
+            if (up)
+                hipLaunchKernelGGL((k_elem_mfma<P, T, true, true>), ge, dim3(64), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_elem_mfma<P, T, false, true>), ge, dim3(64), 0, c->stream, a);
+        } else if (up) {
+            hipLaunchKernelGGL((k_elem_mfma<P, T, true>), ge, dim3(64), 0, c->stream, a);
+        } else {
+            hipLaunchKernelGGL((k_elem_mfma<P, T, false>), ge, dim3(64), 0, c->stream, a);
+        }
         se.stop();
     }
 }
@@ -750,6 +865,8 @@ std::string stage_kernel_names(const fvdg_ctx* c) {
     const bool dpp = P >= 2 && P <= 4 && !c->lds_face_flux;
     const std::string flux = dpp ? "k_face_flux_dpp<" + ps + "," + T + "," + std::to_string(c->flux_threads) + ">"
                                  : "k_face_flux_lds<" + ps + "," + T + ">";
+    if (use_elem(c) && !fused_extrema && use_face_tile(c))
+        return "volume=;flux=k_face_tile<" + ps + "," + T + ">;test=k_elem_mfma<" + ps + "," + T + ",true,true>";
     if (use_elem(c) && !fused_extrema) return "volume=;flux=" + flux + ";test=k_elem_mfma<" + ps + "," + T + ",true>";
     std::string vol = c->precision == FVDG_PREC_BF16X3 ? "k_vol_bf16x3<" + ps + ">"
                                                        : "k_vol_mfma<" + ps + "," + T + (fused_extrema ? ",true>" : ">");
@@ -834,6 +951,8 @@ int fvdg_create(fvdg_ctx** out, int device) {
     const char* fflux = std::getenv("FVDG_FACE_FLUX");
     c->lds_face_flux = fflux && std::strcmp(fflux, "lds") == 0;
     if (const char* gm = std::getenv("FVDG_GRID_MULT")) c->grid_mult = std::max(1, std::atoi(gm));
+    if (const char* ft = std::getenv("FVDG_FACE_TILE"))
+        c->face_tile = std::strcmp(ft, "off") == 0 ? 0 : std::strcmp(ft, "all") == 0 ? 2 : 1;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     const int rc = scratch_init(device);  // the static scratch's address on this device
     if (rc) return rc;
@@ -936,6 +1055,47 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
             cmeta[e * 4 + l] = (uint8_t)(fcombo[2 * f + side] | (Fc.is_boundary() ? 0x80 : 0));
         }
     }
+    // face tiles of k_face_tile: faces grouped by class = side-0 combination * 25 + (side-1
+    // combination, or 24 on the boundary), each class in face order, padded to whole tiles of 16
+    std::vector<int32_t> tface, tclass, trec;
+    {
+        constexpr int NCL = 24 * 25;
+        std::vector<int32_t> fcl(F);
+        std::vector<int64_t> cnt(NCL, 0), start(NCL, 0);
+        for (int64_t f = 0; f < F; ++f) {
+            fcl[f] = fcombo[2 * f] * 25 + (fcell[2 * f + 1] >= 0 ? fcombo[2 * f + 1] : 24);
+            ++cnt[fcl[f]];
+        }
+        int64_t nt = 0;
+        for (int k = 0; k < NCL; ++k) {
+            start[k] = nt * 16;
+            nt += (cnt[k] + 15) / 16;
+        }
+        tface.assign(nt * 16, -1);
+        tclass.assign(nt, 0);
+        for (int k = 0; k < NCL; ++k)
+            for (int64_t t = start[k] / 16; t < start[k] / 16 + (cnt[k] + 15) / 16; ++t) tclass[t] = k;
+        for (int64_t f = 0; f < F; ++f) tface[start[fcl[f]]++] = (int32_t)f;
+        c->n_face_tiles = nt;
+        // column records: face, side-0 cell, side-1 cell, kind | pseudo3dz << 2 | (Dirichlet slot + 1) << 3
+        trec.assign(nt * 16 * 4, -1);
+        for (int64_t i = 0; i < nt * 16; ++i) {
+            const int32_t f = tface[i];
+            int32_t* r = &trec[4 * i];
+            r[3] = 0;
+            if (f < 0) continue;
+            r[0] = f;
+            r[1] = fcell[2 * f];
+            r[2] = fcell[2 * f + 1];
+            if (r[2] >= 0) {
+                r[3] = 1;
+            } else if (fbc[f] == FVDG_BC_DIRICHLET) {
+                r[3] = 2 | ((fdir[f] + 1) << 3);
+            } else if (fbc[f] == FVDG_BC_PSEUDO3DZ) {
+                r[3] = 2 | 4;
+            }
+        }
+    }
     c->E = E;
     c->Epad = (E + 63) / 64 * 64;
     c->F = F;
@@ -953,6 +1113,8 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
     HIP_TRY(c->fgeo.upload(fgeo, c->stream));
     HIP_TRY(c->fbc.upload(fbc, c->stream));
     HIP_TRY(c->fdir.upload(fdir, c->stream));
+    HIP_TRY(c->trec.upload(trec, c->stream));
+    HIP_TRY(c->tclass.upload(tclass, c->stream));
     // keep the Dirichlet faces' corner coordinates to place their quadrature points
     c->dir_xyz.resize(c->n_dir * 9);
     for (int64_t i = 0; i < c->n_dir; ++i)

@@ -131,18 +131,22 @@ def test_fp32_trajectory_tracks_fp64(solver, order, n, steps, prec):
 @pytest.mark.parametrize("prec", MIXED)
 @pytest.mark.parametrize("order,n", [(2, 6), (3, 6), (4, 4)])
 def test_mixed_face_paths_agree(oracle, order, n, prec, monkeypatch):
-    """Every face path of the mixed stage (default, the LDS-staged face-flux pass, the LDS test
-    pass) within 2e-5 of the oracle residual and within 1e-6 of the default path's residual and
-    two-step state."""
+    """Every face path of the mixed stage (default: the face tiles at P2-P4 in FP32; the row-broadcast
+    face-flux pass + element kernel; the LDS-staged face-flux pass; the LDS test pass) within 2e-5 of
+    the oracle residual, within 3e-6 of the default path's residual (FP32 contractions summed in
+    other orders) and within 1e-6 of its two-step state."""
     print(f"[case] mixed face paths {prec} P{order}", flush=True)
     mesh = F.Mesh.box(n, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=5)
     U = mesh.project(order, "vortex", VORTEX)
     U *= 1 + 1e-2 * np.random.default_rng(3).standard_normal(U.shape)
     out = {}
-    for path in ("default", "lds", "lds_flux"):
+    for path in ("default", "no_tile", "lds", "lds_flux"):
         monkeypatch.delenv("FVDG_FACE_TEST", raising=False)
         monkeypatch.delenv("FVDG_FACE_FLUX", raising=False)
-        if path == "lds":
+        monkeypatch.delenv("FVDG_FACE_TILE", raising=False)
+        if path == "no_tile":
+            monkeypatch.setenv("FVDG_FACE_TILE", "off")
+        elif path == "lds":
             monkeypatch.setenv("FVDG_FACE_TEST", "lds")
         elif path == "lds_flux":
             monkeypatch.setenv("FVDG_FACE_FLUX", "lds")
@@ -156,5 +160,5 @@ def test_mixed_face_paths_agree(oracle, order, n, prec, monkeypatch):
     Ro = oracle.residual(mesh.export(), order, U, flux="roe")
     for path, (R, U2) in out.items():
         assert rel_err(R, Ro) < 2e-5, path
-        assert rel_err(R, out["default"][0]) < 1e-6, path
+        assert rel_err(R, out["default"][0]) < 3e-6, path
         assert rel_err(U2, out["default"][1]) < 1e-6, path
