@@ -92,6 +92,7 @@ struct fvdg_ctx {
     // per (face, side) test contributions
     int face_tile = 1;                    // 0 off, 1 mixed precision (FP32), 2 also FP64 (FVDG_FACE_TILE)
     int64_t n_face_tiles = 0;
+    int tile_order = 2;                   // 0 class order, 1 spatial, 2 spatial in 8 XCD ranges (FVDG_TILE_ORDER)
     DevBuf<int32_t> trec, tclass;         // n_face_tiles*16 column records (int4: face or -1, cells, kind),
                                           // n_face_tiles classes
     DevBuf<double> tAt, tAe;              // FP64 operand layouts

@@ -66,7 +66,7 @@ struct ElemShape {
     // waves per SIMD the registers are held to: the tile's LDS (16 x 5N doubles, 12.9 KB at P3) allows
     // three; the FP64 form would spill 456 bytes per lane at three (two), the FP32 form 20 (three:
     // P3 196k 0.434 vs 0.457 ms)
-    static constexpr int MIN_WAVES = (N <= 20 && !std::is_same<T, double>::value) ? 3 : 2;
+    static constexpr int MIN_WAVES = (N <= 10 && !std::is_same<T, double>::value) ? 3 : 2;
     static constexpr bool SUPPORTED = N <= 35;
 };
 
@@ -80,40 +80,55 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
     constexpr int KPAD = S::KPAD, JPAD = S::JPAD, STR = S::STRIDE;
     constexpr int QF = Ord<ORDER>::QF, FKS = (QF + 3) / 4;
     __shared__ double sU[16 * STR];  // the tile's coefficients (FP64), later its stage values (or residual)
-    __shared__ int sPinv[3 * QF];
+    __shared__ int sPinv[CF ? 1 : 3 * QF];
     __shared__ int sCanon[24];
+    __shared__ double sW[16 * QT + N];  // volume weights (zero padded), then mhat
     const int l = threadIdx.x, col = l & 15, grp = l >> 4;
-    for (int i = l; i < 3 * QF; i += 64) {
-        const int m = i / QF, g = i - m * QF;
-        sPinv[m * QF + a.perm[i]] = g;
-    }
-    if (l < 24) sCanon[l] = a.canon[l];
     const int64_t e0 = (int64_t)blockIdx.x * 16;
     const int64_t base = e0 * NV, limit = a.E * NV;
     const bool do_cells = (a.parts & FVDG_PART_CELLS) != 0;
     const bool do_int = (a.parts & FVDG_PART_INTERNALS) != 0, do_bnd = (a.parts & FVDG_PART_BOUNDARYS) != 0;
+    const int64_t e = e0 + col;
+    const bool live = e < a.E;
+    const int64_t ec = live ? e : a.E - 1;  // clamped: every load below is unconditional
 
-    // 1. the tile's coefficients (16 NV contiguous values) in LDS
+    // 1. everything the tile reads up front, issued together: the lane's element geometry and face
+    //    slots, the small tables (LDS), the tile's coefficients (LDS)
+    double Ai[9], det, c0[5] = {0, 0, 0, 0, 0};
+    {
+        const double* g = a.cgeo + ec * 10;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ai[i] = live ? g[i] : 0.0;
+        det = live ? g[9] : 0.0;
+        if (SPLIT)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) c0[v] = live ? a.U[ec * NV + v] : 0.0;
+    }
+    const int4 cf4r = *reinterpret_cast<const int4*>(a.cfaces + ec * 4);
+    const uchar4 cm4r = *reinterpret_cast<const uchar4*>(a.cmeta + ec * 4);
+    {
+        constexpr int NW = 16 * QT + N;
+        const double w0 = a.wq[min(l, 16 * QT - 1)];
+        const double w1 = l + 64 < NW ? (l + 64 < 16 * QT ? a.wq[l + 64] : a.mhat[max(l + 64 - 16 * QT, 0)]) : 0.0;
+        const double m0v = a.mhat[min(max(l - 16 * QT, 0), N - 1)];
+        if (l < 16 * QT) sW[l] = w0;
+        else if (l < NW) sW[l] = m0v;
+        if (l + 64 < NW) sW[l + 64] = w1;
+        if (!CF) {
+            for (int i = l; i < 3 * QF; i += 64) {
+                const int m = i / QF, g = i - m * QF;
+                sPinv[m * QF + a.perm[i]] = g;
+            }
+        }
+        if (l < 24) sCanon[l] = a.canon[l];
+    }
     stage_to_lds<16 * NV, 64>(
         l, [&](int i) { return (base + i < limit) ? a.U[base + i] : 0.0; },
         [&](int i, double x) { sU[(i / NV) * STR + (i % NV)] = x; });
     __syncthreads();
 
-    // 2. geometry of this lane's element (column col); split: mean state and its flux
-    const int64_t e = e0 + col;
-    double Ai[9], det = 0.0, c0[5] = {0, 0, 0, 0, 0}, FrA[5][3];
-    if (e < a.E) {
-        const double* g = a.cgeo + e * 10;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Ai[i] = g[i];
-        det = g[9];
-        if (SPLIT)
-#pragma unroll
-            for (int v = 0; v < 5; ++v) c0[v] = a.U[e * NV + v];
-    } else {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Ai[i] = 0.0;
-    }
+    // 2. split: the mean state's flux times A^{-1}
+    double FrA[5][3];
     if (SPLIT) {
         double F[5][3];
         euler_flux(c0, a.gamma, F);
@@ -133,31 +148,49 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
     if (do_cells) {
 #pragma unroll 1
         for (int t = 0; t < QT; ++t) {
+            // this point tile's interpolation operands and the first point row's test operands
+            T va[KS], ga[3][JT];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) va[s] = a.VA[(16 * t + col) * KPAD + 4 * s + grp];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt) ga[d][jt] = a.GA[((16 * t + M::row(0, grp)) * 3 + d) * JPAD + 16 * jt + col];
             acc_t uacc[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) uacc[v] = (acc_t){0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                const T av = a.VA[(16 * t + col) * KPAD + 4 * s + grp];
-                const int k = 4 * s + grp;
+                const int k = 4 * s + grp, kc = min(k, N - 1);
+                const bool use_k = k < N && (!SPLIT || k > 0);  // split: mode 0 is added in FP64
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    // split: mode 0 (the cell mean) is added in FP64 at the points
-                    const T bv = (k < N && (!SPLIT || k > 0)) ? (T)sU[col * STR + 5 * k + v] : (T)0;
-                    uacc[v] = M::mma(av, bv, uacc[v]);
+                    const double x = sU[col * STR + 5 * kc + v];
+                    uacc[v] = M::mma(va[s], use_k ? (T)x : (T)0, uacc[v]);
                 }
+                asm volatile("" ::: "memory");  // one K step's LDS reads in flight at a time
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int q = 16 * t + M::row(r, grp);
+                // the next point row's test operands, in flight during this row's flux
+                T gn[3][JT];
+                if (r < 3) {
+                    const int qn = 16 * t + M::row(r + 1, grp);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int jt = 0; jt < JT; ++jt) gn[d][jt] = a.GA[(qn * 3 + d) * JPAD + 16 * jt + col];
+                }
                 T B[5][3];
-                if (q < Q) {
+                {
+                    // every lane evaluates (rows q >= Q have zero weight and zero test operands)
                     double Uq[5];
 #pragma unroll
                     for (int v = 0; v < 5; ++v) Uq[v] = c0[v] + (double)uacc[v][r];
                     double F[5][3];
                     euler_flux(Uq, a.gamma, F);
-                    const double jw = -a.wq[q] * det;
+                    const double jw = -sW[q] * det;
 #pragma unroll
                     for (int v = 0; v < 5; ++v)
 #pragma unroll
@@ -166,32 +199,27 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                             if (SPLIT) fa -= FrA[v][d];
                             B[v][d] = (T)(fa * jw);
                         }
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 5; ++v)
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) B[v][d] = (T)0;
                 }
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
 #pragma unroll
                     for (int jt = 0; jt < JT; ++jt) {
-                        const T av = a.GA[(q * 3 + d) * JPAD + 16 * jt + col];
 #pragma unroll
-                        for (int v = 0; v < 5; ++v) racc[v][jt] = M::mma(av, B[v][d], racc[v][jt]);
+                        for (int v = 0; v < 5; ++v) racc[v][jt] = M::mma(ga[d][jt], B[v][d], racc[v][jt]);
                     }
                 }
+                if (r < 3)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int jt = 0; jt < JT; ++jt) ga[d][jt] = gn[d][jt];
             }
         }
     }
 
     // 4. face test: the four slots of this lane's element (column col)
-    int4 cf4 = make_int4(-1, -1, -1, -1);
-    uchar4 cm4 = make_uchar4(0, 0, 0, 0);
-    if (e < a.E) {
-        cf4 = *reinterpret_cast<const int4*>(a.cfaces + e * 4);
-        cm4 = *reinterpret_cast<const uchar4*>(a.cmeta + e * 4);
-    }
+    const int4 cf4 = live ? cf4r : make_int4(-1, -1, -1, -1);
+    const uchar4 cm4 = live ? cm4r : make_uchar4(0, 0, 0, 0);
     int slot_c[4] = {-1, -1, -1, -1};  // split: each slot's table and face (-f-1: side 1), for step 5
     int64_t slot_f[4] = {0, 0, 0, 0};
     // CF, split: the cell-mean row (j = 0, entry 0 of lane group 0) from the faces' FP64 flux
@@ -348,7 +376,7 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                 }
                 double* p = sU + col * STR + 5 * j;
                 if (FUSE) {
-                    const double inv_m = e < a.E ? rcp_pos(det * a.mhat[j]) : 0.0;
+                    const double inv_m = e < a.E ? rcp_pos(det * sW[16 * QT + j]) : 0.0;
 #pragma unroll
                     for (int v = 0; v < 5; ++v) p[v] = p[v] + dte * (-rr[v] * inv_m);
                 } else {

@@ -110,7 +110,9 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
     const int32_t f = rec.x;
     const int kind = rec.w & 3;
     if (grp == 0) {
-        sCell[col] = kind != 0 ? rec.y : -1;
+        // side 0's cell of every face (a boundary face without a flux still takes its cell's mean
+        // flux reference, which the element kernel leaves out of the cell's other terms)
+        sCell[col] = f >= 0 ? rec.y : -1;
         sCell[16 + col] = kind == 1 ? rec.z : -1;
     }
     // geometry and weights of the lane's face and points, needed after the traces
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
 
     // 3. Riemann solves at the lane's six points
     const double n[3] = {g01.x, g01.y, g23.x};
-    const double J = kind != 0 ? g23.y : 0.0;
+    const double J = f >= 0 ? g23.y : 0.0;
     const bool pseudo = (rec.w & 4) != 0;
     const int32_t dslot = (rec.w >> 3) - 1;
     double m0[5] = {0, 0, 0, 0, 0}, m1[5] = {0, 0, 0, 0, 0};
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
     if (SPLIT) {
         euler_flux_n(euler_prim(m0, a.gamma), n, ref0);
         if (two) euler_flux_n(euler_prim(m1, a.gamma), n, ref1);
-        if (kind == 0)
+        if (f < 0)
 #pragma unroll
             for (int v = 0; v < 5; ++v) ref0[v] = 0.0;
     }

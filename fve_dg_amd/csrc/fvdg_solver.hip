@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -951,6 +952,8 @@ int fvdg_create(fvdg_ctx** out, int device) {
     const char* fflux = std::getenv("FVDG_FACE_FLUX");
     c->lds_face_flux = fflux && std::strcmp(fflux, "lds") == 0;
     if (const char* gm = std::getenv("FVDG_GRID_MULT")) c->grid_mult = std::max(1, std::atoi(gm));
+    if (const char* to = std::getenv("FVDG_TILE_ORDER"))
+        c->tile_order = std::strcmp(to, "class") == 0 ? 0 : std::strcmp(to, "space") == 0 ? 1 : 2;
     if (const char* ft = std::getenv("FVDG_FACE_TILE"))
         c->face_tile = std::strcmp(ft, "off") == 0 ? 0 : std::strcmp(ft, "all") == 0 ? 2 : 1;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1076,6 +1079,29 @@ int fvdg_set_mesh(fvdg_ctx* c, const fvdg_mesh* m) {
         for (int k = 0; k < NCL; ++k)
             for (int64_t t = start[k] / 16; t < start[k] / 16 + (cnt[k] + 15) / 16; ++t) tclass[t] = k;
         for (int64_t f = 0; f < F; ++f) tface[start[fcl[f]]++] = (int32_t)f;
+        if (c->tile_order > 0) {
+            // launch order: tiles by their smallest side-0 cell, so that tiles of different classes
+            // over the same cells run together and share the cells' coefficient blocks in L2;
+            // tile_order 2 also splits that order into 8 contiguous ranges, one per XCD (block b
+            // runs on XCD b % 8)
+            std::vector<int64_t> key(nt, INT64_MAX), ord(nt);
+            for (int64_t t = 0; t < nt; ++t) {
+                ord[t] = t;
+                for (int i = 0; i < 16; ++i)
+                    if (tface[16 * t + i] >= 0) key[t] = std::min<int64_t>(key[t], fcell[2 * tface[16 * t + i]]);
+            }
+            std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return key[x] < key[y]; });
+            const int64_t ntp = c->tile_order == 2 ? (nt + 7) / 8 * 8 : nt, chunk = ntp / 8;
+            std::vector<int32_t> tf2(ntp * 16, -1), tc2(ntp, 0);
+            for (int64_t p = 0; p < nt; ++p) {
+                const int64_t b = c->tile_order == 2 ? (p % chunk) * 8 + p / chunk : p;
+                tc2[b] = tclass[ord[p]];
+                for (int i = 0; i < 16; ++i) tf2[16 * b + i] = tface[16 * ord[p] + i];
+            }
+            tface.swap(tf2);
+            tclass.swap(tc2);
+            nt = ntp;
+        }
         c->n_face_tiles = nt;
         // column records: face, side-0 cell, side-1 cell, kind | pseudo3dz << 2 | (Dirichlet slot + 1) << 3
         trec.assign(nt * 16 * 4, -1);

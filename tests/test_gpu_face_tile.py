@@ -126,6 +126,32 @@ def test_face_tile_parts_with_boundaries(tile_all, oracle, prec, order):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("order", [2, 3])
+def test_face_tile_flux_free_boundary_faces(tile_all, oracle, prec, order):
+    """Boundary types without an explicit face flux (Neumann = 1, Outflow = 7: they contribute
+    nothing) beside Dirichlet faces.  FP32 takes every face difference from the cell's mean flux,
+    so a flux-free face must still carry that reference: a cell touching one is checked here."""
+    print(f"[case] face tile flux-free faces {prec} P{order}", flush=True)
+    mesh = F.Mesh.box((3, 3, 2), ub=(10, 10, 5), split=6, jitter=0.1, seed=7)
+    bc = mesh.boundary_types()
+    bnd = mesh.export()["face_cells"][:, 1] < 0
+    idx = np.flatnonzero(bnd)
+    bc[idx[0::3]] = 1
+    bc[idx[1::3]] = 7
+    mesh.set_boundary_types(bc)
+    params = [1.4, 5.0, 5.0, 2.5, 1.0, 0.5, 0.0, 0.0]
+    U = mesh.project(order, "vortex", params)
+    tile_all.set_scheme(order, flux="roe", precision=prec).set_mesh(mesh)
+    tile_all.upload(U)
+    _, xyz = tile_all.dirichlet_points()
+    UD = np.array([[oracle.vortex_state(params, 0.0, x) for x in row] for row in xyz])
+    tile_all.set_dirichlet_states(UD)
+    Rm = tile_all.residual()
+    Ro = oracle.residual(mesh.export(), order, U, flux="roe", bc=mesh.boundary_types(), UD=UD)
+    assert rel_err(Rm, Ro) < (1e-12 if prec == "fp64" else 2e-5)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("order", [2, 3])
 def test_face_tile_fluent_mesh(tile_all, oracle, prec, order):
     """The reference's case11.msh: unstructured, faces in every vertex-combination class."""
     print(f"[case] face tile fluent {prec} P{order}", flush=True)
