@@ -70,6 +70,35 @@ struct ElemShape {
     static constexpr bool SUPPORTED = N <= 35;
 };
 
+// F(U0 + d) - F(U0) for the Euler flux, from the primitives of U0 (pb: rho0, u0 (3), E0 = rho0 e0,
+// p0) and the perturbation d = (d rho, d m (3), d E) = entry r of the accumulators u[v], in FP32
+// with every term proportional to d (no difference of two large fluxes):
+//   du = (dm - u0 d rho) / rho,  u = u0 + du,  dp = (gamma - 1) (dE - (dm . u + rho0 u0 . du) / 2)
+//   mass: dm_d;  momentum: dm_i u_d + rho0 u0_i du_d + dp [i = d];  energy: (dE + dp) u_d + (E0 + p0) du_d
+template <typename A4>
+__device__ __forceinline__ void flux_delta_f32(const float pb[6], const A4 (&u)[5], int r, float gm1, float F[5][3]) {
+    const float dr = u[0][r], dm[3] = {u[1][r], u[2][r], u[3][r]}, dE = u[4][r];
+    const float r0 = pb[0], u0[3] = {pb[1], pb[2], pb[3]};
+    const float ri = __builtin_amdgcn_rcpf(r0 + dr);
+    float du[3], uu[3], dmu = 0.0f, u0du = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        du[d] = (dm[d] - u0[d] * dr) * ri;
+        uu[d] = u0[d] + du[d];
+        dmu += dm[d] * uu[d];
+        u0du += u0[d] * du[d];
+    }
+    const float dp = gm1 * (dE - 0.5f * (dmu + r0 * u0du));
+    const float dEp = dE + dp, Ep0 = pb[4] + pb[5];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        F[0][d] = dm[d];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) F[1 + i][d] = dm[i] * uu[d] + r0 * u0[i] * du[d] + (i == d ? dp : 0.0f);
+        F[4][d] = dEp * uu[d] + Ep0 * du[d];
+    }
+}
+
 template <int ORDER, typename T, bool FUSE, bool CF = false>
 __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_mfma(ElemArgs<T> a) {
     using S = VolShape<ORDER>;
@@ -127,9 +156,25 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
         [&](int i, double x) { sU[(i / NV) * STR + (i % NV)] = x; });
     __syncthreads();
 
-    // 2. split: the mean state's flux times A^{-1}
+    // 2. split: the mean state's flux times A^{-1}; CF: the mean state's primitives in FP32 for the
+    //    perturbation flux (PF: every point's flux is formed as its difference from the mean's)
+    constexpr bool PF = CF && SPLIT;
     double FrA[5][3];
-    if (SPLIT) {
+    float pb[6] = {1, 0, 0, 0, 0, 0}, Af[9], detf = 0.0f;  // rho0, u0 (3), E0, p0
+    if (PF) {
+        const Prim q0 = euler_prim(c0, a.gamma);
+        if (live) {
+            pb[0] = (float)q0.rho;
+            pb[1] = (float)q0.u;
+            pb[2] = (float)q0.v;
+            pb[3] = (float)q0.w;
+            pb[4] = (float)c0[4];
+            pb[5] = (float)q0.p;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Af[i] = (float)Ai[i];
+        detf = (float)det;
+    } else if (SPLIT) {
         double F[5][3];
         euler_flux(c0, a.gamma, F);
 #pragma unroll
@@ -183,7 +228,18 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                         for (int jt = 0; jt < JT; ++jt) gn[d][jt] = a.GA[(qn * 3 + d) * JPAD + 16 * jt + col];
                 }
                 T B[5][3];
-                {
+                if constexpr (PF) {
+                    // F(U_q) - F(c_0) from the perturbation d = U_q - c_0 (the FP32 interpolation of
+                    // modes >= 1) and the mean's primitives, without cancellation: all FP32
+                    float dF[5][3];
+                    flux_delta_f32(pb, uacc, r, (float)(a.gamma - 1.0), dF);
+                    const float jw = -(float)sW[q] * detf;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                            B[v][d] = (T)((dF[v][0] * Af[d] + dF[v][1] * Af[3 + d] + dF[v][2] * Af[6 + d]) * jw);
+                } else {
                     // every lane evaluates (rows q >= Q have zero weight and zero test operands)
                     double Uq[5];
 #pragma unroll
