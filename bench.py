@@ -27,7 +27,9 @@ face points):
          three stages, cell and face geometry)
 per kernel: the face-flux kernel F = 40QfN + 410Qf, B = 40N + 88 (its stage input and the face
 geometry); the element kernel (volume + face test + update) F = 40QN + 145Q + 40QfN + 25N,
-B = 40N (2 + 2/3) + 80.  `roofline` is the dominant (longest) kernel; `traffic` is its HBM bytes
+B = 40N (2 + 2/3) + 80.  With face tiles (the mixed path at P2-P4) the face test moves from the
+element kernel to the face pass: face pass F = 80QfN + 410Qf, element kernel F = 40QN + 145Q +
+25N, bytes as above.  `roofline` is the dominant (longest) kernel; `traffic` is its HBM bytes
 per launch measured by rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a device copy)
 for this workload, read from profiles/hbm_traffic_<workload>.txt (null when absent).  `stage`
 gives the same counts for the whole stage, with the measured bytes beside the algorithmic ones.
@@ -109,6 +111,11 @@ def roofline(ms, names, E, Q, Qf, N, precision, traffic):
     """Per-kernel and per-stage roofline of one precision's stage (ms from fvdg_time_stages)."""
     c = counts(Q, Qf, N)
     fused = not names.get("volume")
+    # face tiles: the face pass also runs the face test, the element kernel the volume and update
+    tiles = (names.get("flux") or "").startswith("k_face_tile")
+    if tiles:
+        c["flux"] = (c["flux"][0] + 40 * Qf * N, c["flux"][1])
+        c["elem"] = (c["elem"][0] - 40 * Qf * N, c["elem"][1])
     slots = [("volume", ms[1]), ("flux", ms[2]), ("elem" if fused else "test", ms[3])]
     label = {"volume": "volume", "flux": "flux", "test": "test", "elem": "test"}
     kernels = {}
@@ -120,8 +127,8 @@ def roofline(ms, names, E, Q, Qf, N, precision, traffic):
         if precision == "fp64":
             peak_f = PEAK_FP64
         else:  # contractions at the FP32 matrix peak, the rest (pointwise, Riemann, update) at FP64
-            Fc = {"volume": 40 * Q * N, "flux": 40 * Qf * N, "test": 40 * Qf * N,
-                  "elem": 40 * Q * N + 40 * Qf * N}[slot]
+            Fc = {"volume": 40 * Q * N, "flux": (80 if tiles else 40) * Qf * N, "test": 40 * Qf * N,
+                  "elem": 40 * Q * N + (0 if tiles else 40 * Qf * N)}[slot]
             peak_f = F / (Fc / PEAK_FP32 + (F - Fc) / PEAK_FP64)
         t_s = t * 1e-3
         tf, tb = F * E / peak_f, B * E / PEAK_HBM
@@ -305,8 +312,10 @@ def run_single(args, w):
                    "wall_s": round(f64["wall"], 3), "kernels": f64["names"],
                    "analytic_l2_rel": analytic(f64), "after_steps": steps_done},
         "roofline": roofline(f64["ms"], f64["names"], E, Q, Qf, N, "fp64", traffic),
-        "mixed": mixed_block("fp32", "FP32 matrix-core contractions (FP32 accumulation), FP32 face traces, FP64 "
-                                     "state / pointwise flux / Riemann central flux and pressure / update / sums"),
+        "mixed": mixed_block("fp32", "FP32 matrix-core contractions (volume interpolation and test, face traces "
+                                     "and face test; FP32 accumulation) on differences from each cell's mean, FP32 "
+                                     "volume flux perturbation; FP64 Riemann central flux, pressures and positivity "
+                                     "(FP32 upwind dissipation), face flux integrals, state, update and sums"),
         "mixed_bf16x3": mixed_block("bf16x3", "volume contractions as 6 bf16 matrix-core products of 3-part splits "
                                               "(FP32 accumulation), face contractions FP32, FP64 state and fluxes"),
         "limited": {"limiter": "PositiveLimiter apply_1 + apply_2 after every stage (fp64)",
