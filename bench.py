@@ -363,6 +363,8 @@ def run_ensemble(args, w):
         if prec == "fp64":
             print(f"[bench] ensemble: {P} problems, {tets} tets, {dofs} DOF, setup "
                   f"{time.perf_counter() - t_setup:.1f}s, autotune {tuned}", file=sys.stderr, flush=True)
+        else:
+            print(f"[bench] ensemble fp32 autotune {tuned}", file=sys.stderr, flush=True)
         if args.warmup > 0:
             ens.step(args.warmup, graph=True)
         ms = ens.time_steps(args.steps, graph=True)
@@ -393,7 +395,8 @@ def run_ensemble(args, w):
                    "hexes_per_axis": [int(ns.min()), int(ns.max())], "stages_per_step": 3, "hip_graph": True,
                    "concurrent_order_groups": True,
                    "serial_groups_ms_per_step": round(res["fp64"]["serial_ms"], 4),
-                   "autotuned": {str(k): v for k, v in res["fp64"]["tuned"].items()}},
+                   "autotuned": {str(k): v for k, v in res["fp64"]["tuned"].items()},
+                   "autotuned_fp32": {str(k): v for k, v in res["fp32"]["tuned"].items()}},
         "roofline": None,
         "mixed": {"precision": "fp32 contractions, fp64 fluxes/update/state", "value": round(res["fp32"]["value"], 1),
                   "ms_per_step": round(res["fp32"]["ms"], 4),

@@ -557,12 +557,13 @@ class Ensemble:
         return ms.value
 
     def autotune(self, trial_steps=2):
-        """per order: {'elem_fused': bool, 'face_flux': 'dpp' | 'lds', 'flux_threads': block size}"""
+        """per order: {'elem_fused': bool, 'face_flux': 'tile' | 'dpp' | 'lds', 'flux_threads': block size}"""
         ng = len(self.groups())
         ch = (C.c_int * (2 * ng))()
         _check(lib().fvdg_ensemble_autotune(self._h, trial_steps, ch))
-        return {o: {"elem_fused": bool(ch[2 * i]), "face_flux": "dpp" if ch[2 * i + 1] else "lds",
-                    "flux_threads": ch[2 * i + 1] or None}
+        kinds = {-1: "tile", 0: "lds"}
+        return {o: {"elem_fused": bool(ch[2 * i]), "face_flux": kinds.get(ch[2 * i + 1], "dpp"),
+                    "flux_threads": ch[2 * i + 1] if ch[2 * i + 1] > 0 else None}
                 for i, o in enumerate(self.groups())}
 
     def set_serial(self, serial=True):

@@ -11,9 +11,11 @@
 // becomes its branches) and replayed per step.
 //
 // The autotuner times, per order group on the ensemble's own data, the stage's kernel
-// choices: the fused element kernel or the volume + face-test kernels, the row-broadcast
-// face-flux kernel in 128-, 256- or 512-thread blocks (16 faces per 256 threads) or the
-// LDS-staged one, keeping the fastest; the state is restored after every trial.
+// choices: for FP32 groups at P2-P4 the face tiles (16 faces of one vertex-combination class per
+// wavefront, both face contractions on the matrix cores); the fused element kernel or the volume +
+// face-test kernels, with the row-broadcast face-flux kernel in 128-, 256- or 512-thread blocks
+// (16 faces per 256 threads) or the LDS-staged one; keeping the fastest; the state is restored
+// after every trial.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -321,18 +323,24 @@ int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices) {
         DevBuf<double> save;
         HIP_TRY(save.alloc(nd));
         HIP_TRY(hipMemcpyAsync(save.p, c->U.p, nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        // candidates: (fused element kernel, face-flux kernel: 0 = LDS-staged, else row-broadcast block size)
+        // candidates: (fused element kernel, face pass: -1 = face tiles, 0 = LDS-staged face flux, else
+        // the row-broadcast face flux's block size)
         const bool dpp_ok = c->order >= 2 && c->order <= 4;
         const bool elem_ok = c->order <= 4 && c->precision != FVDG_PREC_BF16X3;
+        const bool tile_ok = elem_ok && c->precision == FVDG_PREC_FP32 && c->n_face_tiles > 0 && c->tab.Qf == 24 &&
+                             c->order >= 2;
         double best = 1e300;
         int best_elem = c->elem_fused, best_flux = c->lds_face_flux ? 0 : c->flux_threads;
+        const int tile0 = c->face_tile;
         for (int elem = elem_ok ? 1 : 0; elem >= 0; --elem)
-            for (int flux : {256, 128, 512, 0}) {
-                if (flux != 0 && !dpp_ok) continue;
+            for (int flux : {-1, 256, 128, 512, 0}) {
+                if (flux == -1 && (!tile_ok || elem == 0)) continue;
+                if (flux > 0 && !dpp_ok) continue;
                 if (flux == 0 && dpp_ok && elem == 0) continue;  // the split path with the LDS flux: not competitive
                 c->elem_fused = elem != 0;
+                c->face_tile = flux == -1 ? 1 : 0;
                 c->lds_face_flux = flux == 0;
-                c->flux_threads = flux != 0 ? flux : 256;
+                c->flux_threads = flux > 0 ? flux : 256;
                 float t = 0;
                 int rc = time_group(c, trial_steps, &t);
                 if (rc) return rc;
@@ -340,8 +348,9 @@ int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices) {
                 HIP_TRY(hipMemcpyAsync(c->U.p, save.p, nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             }
         c->elem_fused = best_elem != 0;
+        c->face_tile = best_flux == -1 ? 1 : (tile_ok ? 0 : tile0);
         c->lds_face_flux = best_flux == 0;
-        c->flux_threads = best_flux != 0 ? best_flux : 256;
+        c->flux_threads = best_flux > 0 ? best_flux : 256;
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (choices) {
             choices[2 * gi] = best_elem;

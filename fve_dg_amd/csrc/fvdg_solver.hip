@@ -292,9 +292,14 @@ int precision_buffers(fvdg_ctx* c) {
         if (c->Fh32.n != nfh) HIP_TRY(c->Fh32.alloc(nfh));
         if (c->Fref.n != (size_t)c->F * 5) HIP_TRY(c->Fref.alloc((size_t)c->F * 5));
     }
-    // face-tile contributions, allocated when the path serves this precision
+    // face-tile contributions, allocated when the path can serve this precision (an ensemble's
+    // autotuner switches the path on and off per order group)
     const size_t ncf = (size_t)c->F * 2 * 5 * ((c->tab.N + 3) / 4 * 4);
-    if (use_face_tile(c)) {
+    const int ft = c->face_tile;
+    if (c->precision == FVDG_PREC_FP32 && c->face_tile == 0) c->face_tile = 1;
+    const bool tiles = use_face_tile(c);
+    c->face_tile = ft;
+    if (tiles) {
         if (c->precision == FVDG_PREC_FP64) {
             if (c->Cf.n != ncf) HIP_TRY(c->Cf.alloc(ncf));
         } else if (c->Cf32.n != ncf) {

@@ -305,10 +305,10 @@ int fvdg_ensemble_step(fvdg_ensemble* e, int n_steps, int use_graph);
 /* average ms per ensemble step over n_steps (after one untimed step) */
 int fvdg_ensemble_time_steps(fvdg_ensemble* e, int n_steps, int use_graph, double* ms_per_step);
 /* per order group, time the stage's kernel choices over trial_steps steps on the ensemble's own
- * state and keep the fastest (state restored): the fused element kernel or the volume + face-test
- * kernels, and the face-flux kernel (row-broadcast in 128-, 256- or 512-thread blocks at P2-P4, or
- * LDS-staged); choices[2g] = 1 for the fused element kernel, choices[2g+1] = the row-broadcast
- * kernel's block size (0: LDS-staged) */
+ * state and keep the fastest (state restored): the face tiles (FP32 groups at P2-P4), the fused
+ * element kernel or the volume + face-test kernels, and the face-flux kernel (row-broadcast in
+ * 128-, 256- or 512-thread blocks at P2-P4, or LDS-staged); choices[2g] = 1 for the fused element
+ * kernel, choices[2g+1] = the row-broadcast kernel's block size (0: LDS-staged, -1: face tiles) */
 int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices);
 /* serial != 0: every group on the ensemble's stream one after another (for comparison); default:
  * each group on its own stream, forked from and joined back into the ensemble's stream per step */
