@@ -296,3 +296,37 @@ def test_reference_form_limiter_near_vacuum(oracle):
     assert finite and n == 60
     n, finite = _oracle_limited_rarefaction(oracle, 2.0, 60)
     assert not finite and n <= 60
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [2, 3])
+def test_gpu_stage_limiter_mixed_precision(solver, oracle, order):
+    """The stage limiter after FP32-contraction stages (face tiles at P2-P3): one limited SSP-RK3
+    step within 2e-5 of the oracle's limited FP64 step (the FP32 residual's bar)."""
+    print(f"[case] stage limiter fp32 P{order}", flush=True)
+    mesh = MESHES["periodic"]()
+    m = mesh.export()
+    U0 = mesh.project(order, "vortex", VORTEX[:2] + [1.0, 1.0, 0.5, 0.5, 2.0, 2.0])
+    U0[:, 5:] *= 1.5  # sharpen so that the limiter acts
+    dt = 2e-3
+    Minv = 1.0 / oracle.mass(m, order)
+
+    def L(U):
+        return -Minv * oracle.residual(m, order, U, flux="roe")
+
+    def lim(U, prev):
+        return oracle.limiter_apply(order, U, oracle.limiter_bounds(m, order, prev), parts=3)
+
+    U1 = lim(U0 + dt * L(U0), U0)
+    U2 = lim(0.75 * U0 + 0.25 * (U1 + dt * L(U1)), U1)
+    exp = lim(U0 / 3 + 2.0 / 3 * (U2 + dt * L(U2)), U2)
+    assert rel_err(exp, U0 + dt * L(U0)) > 1e-6  # the limiter changed something
+    solver.set_scheme(order, flux="roe", precision="fp32").set_mesh(mesh)
+    solver.upload(U0)
+    solver.set_stage_limiter(3)
+    try:
+        solver.ssprk3(dt, 1)
+    finally:
+        solver.set_stage_limiter(0)
+    assert rel_err(solver.download(), exp) < 2e-5
+    solver.set_scheme(order, flux="roe", precision="fp64")
