@@ -358,33 +358,34 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
         unsigned used = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) used |= (__ballot(cl == c) != 0 ? 1u : 0u) << c;
-        // per table in use: the lanes of elements on that table load their face fluxes (every lane
-        // exactly once per slot), all lanes the table's operands; loads of all K steps are issued
-        // before the products
+        // the lane's face fluxes of this slot, loaded once and unconditionally (clamped point,
+        // face 0 for an unused slot) so the loads issue together instead of behind a branch; the
+        // products below select them
+        T xs[FKS][5];
+#pragma unroll
+        for (int ks = 0; ks < FKS; ++ks) {
+            const T* p = a.Fh + (f * QF + sPinv[m * QF + min(4 * ks + grp, QF - 1)]) * 5;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) xs[ks][v] = sign * p[v];
+        }
+        // per table in use: the table's operands (all lanes), the products with the fluxes of the
+        // lanes whose elements use that table (zero elsewhere)
         for (unsigned u = used; u != 0; u &= u - 1) {
             const int c = __builtin_ctz(u);
             const bool mine = cl == c;
-            T x[FKS][5], b[FKS][JT];
+            T b[FKS][JT];
 #pragma unroll
-            for (int ks = 0; ks < FKS; ++ks) {
-                const int gc = 4 * ks + grp;
-                if (mine && gc < QF) {
-                    const T* p = a.Fh + (f * QF + sPinv[m * QF + gc]) * 5;
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) x[ks][v] = sign * p[v];
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) x[ks][v] = 0;
-                }
+            for (int ks = 0; ks < FKS; ++ks)
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt) b[ks][jt] = a.PhiB[((c * FKS + ks) * JT + jt) * 64 + l];
-            }
 #pragma unroll
             for (int ks = 0; ks < FKS; ++ks)
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
 #pragma unroll
-                    for (int jt = 0; jt < JT; ++jt) racc[v][jt] = M::mma(b[ks][jt], x[ks][v], racc[v][jt]);
+                    for (int jt = 0; jt < JT; ++jt)
+                        racc[v][jt] = M::mma(b[ks][jt], (mine && 4 * ks + grp < QF) ? xs[ks][v] : (T)0,
+                                             racc[v][jt]);
         }
     }
 
