@@ -358,9 +358,15 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
         unsigned used = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) used |= (__ballot(cl == c) != 0 ? 1u : 0u) << c;
-        // the lane's face fluxes of this slot, loaded once and unconditionally (clamped point,
-        // face 0 for an unused slot) so the loads issue together instead of behind a branch; the
-        // products below select them
+        // the first table's operands (all lanes), then the lane's face fluxes of this slot, loaded
+        // once and unconditionally (clamped point, face 0 for an unused slot): all of the slot's
+        // loads in flight together; the products below select the fluxes
+        const int cfirst = used != 0 ? __builtin_ctz(used) : 0;
+        T b[FKS][JT];
+#pragma unroll
+        for (int ks = 0; ks < FKS; ++ks)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) b[ks][jt] = a.PhiB[((cfirst * FKS + ks) * JT + jt) * 64 + l];
         T xs[FKS][5];
 #pragma unroll
         for (int ks = 0; ks < FKS; ++ks) {
@@ -368,16 +374,16 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
 #pragma unroll
             for (int v = 0; v < 5; ++v) xs[ks][v] = sign * p[v];
         }
-        // per table in use: the table's operands (all lanes), the products with the fluxes of the
-        // lanes whose elements use that table (zero elsewhere)
+        // per table in use (the first already loaded): the products with the fluxes of the lanes
+        // whose elements use that table (zero elsewhere)
         for (unsigned u = used; u != 0; u &= u - 1) {
             const int c = __builtin_ctz(u);
             const bool mine = cl == c;
-            T b[FKS][JT];
+            if (c != cfirst)
 #pragma unroll
-            for (int ks = 0; ks < FKS; ++ks)
+                for (int ks = 0; ks < FKS; ++ks)
 #pragma unroll
-                for (int jt = 0; jt < JT; ++jt) b[ks][jt] = a.PhiB[((c * FKS + ks) * JT + jt) * 64 + l];
+                    for (int jt = 0; jt < JT; ++jt) b[ks][jt] = a.PhiB[((c * FKS + ks) * JT + jt) * 64 + l];
 #pragma unroll
             for (int ks = 0; ks < FKS; ++ks)
 #pragma unroll
@@ -444,19 +450,38 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
         }
     __syncthreads();
 
-    // 6. out = a0 U0 + b0 (stage value), or R, over the tile in contiguous runs
-    bool badl = false;
+    // 6. out = a0 U0 + b0 (stage value), or R, over the tile in contiguous runs (U^n, for
+    //    a0 != 0: the loads of CH runs issued together, clamped to the tile, then their outputs)
+    constexpr int NIT = (16 * NV + 63) / 64, CH = 8;
+    // the lane index redefined here: the unrolled runs' index arithmetic is not hoisted above the
+    // barrier (where it would hold registers through the whole kernel)
+    int lo = l;
+    asm volatile("" : "+v"(lo));
     const int64_t nvalid = max<int64_t>(0, min<int64_t>(16, a.E - e0)) * NV;
-    for (int i = l; i < nvalid; i += 64) {
-        const int ee = i / NV, kv = i - ee * NV;
-        const int64_t gi = base + i;
-        const double x = sU[ee * STR + kv];
-        if (FUSE) {
-            const double o = (a.a0 == 0.0) ? a.b0 * x : a.a0 * a.U0[gi] + a.b0 * x;
-            a.out[gi] = o;
-            badl |= stage_value_bad(o, kv);
-        } else {
-            a.R[gi] = x;
+    const bool need_u0 = FUSE && a.a0 != 0.0;
+    bool badl = false;
+#pragma unroll
+    for (int i0 = 0; i0 < NIT; i0 += CH) {
+        double u0v[CH];
+        if (need_u0)
+#pragma unroll
+            for (int k = 0; k < CH && i0 + k < NIT; ++k)
+                u0v[k] = a.U0[base + min<int64_t>(lo + 64 * (i0 + k), nvalid - 1)];
+#pragma unroll
+        for (int k = 0; k < CH && i0 + k < NIT; ++k) {
+            const int i = lo + 64 * (i0 + k);
+            if (i < nvalid) {
+                const int ee = i / NV, kv = i - ee * NV;
+                const int64_t gi = base + i;
+                const double x = sU[ee * STR + kv];
+                if (FUSE) {
+                    const double o = need_u0 ? a.a0 * u0v[k] + a.b0 * x : a.b0 * x;
+                    a.out[gi] = o;
+                    badl |= stage_value_bad(o, kv);
+                } else {
+                    a.R[gi] = x;
+                }
+            }
         }
     }
     if (FUSE && badl && a.bad) a.bad[0] = 1;
