@@ -247,6 +247,26 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) ref0[v] = 0.0;
     }
+    // the test operands of both sides and side 1's mean-row correction, in flight during the solves
+    // (FP32 at P2, P3; otherwise each side's operands are loaded at its products: registers)
+    constexpr bool PRE = SPLIT && JT <= 2;
+    const int cbs1 = two ? cb1 : cb0;
+    T ae[2][JT][6], icv[JT][4];
+    auto load_ae = [&](int s) __attribute__((always_inline)) {
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+            for (int ks = 0; ks < 6; ++ks) ae[s][jt][ks] = a.Ae[((size_t)(s ? cbs1 : cb0) * JT * 6 + jt * 6 + ks) * 64 + l];
+    };
+    if (PRE) {
+        load_ae(0);
+        load_ae(1);
+    }
+    if (SPLIT)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) icv[jt][r] = a.ICc[(size_t)cbs1 * NP + min(16 * jt + LY::jrow(r, grp), NP - 1)];
     T dF[6][5];
     static_for<6>([&](auto SC) __attribute__((always_inline)) {
         constexpr int s = decltype(SC)::value;
@@ -276,7 +296,7 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         if (s == 1 && !two) break;
-        const T* Ae = a.Ae + (size_t)(s ? cb1 : cb0) * JT * 6 * 64;
+        if (!PRE) load_ae(s);
         acc_t te[5][JT];
 #pragma unroll
         for (int v = 0; v < 5; ++v)
@@ -286,23 +306,18 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
         for (int ks = 0; ks < 6; ++ks)
 #pragma unroll
             for (int jt = 0; jt < JT; ++jt) {
-                const T av = Ae[(jt * 6 + ks) * 64 + l];
 #pragma unroll
-                for (int v = 0; v < 5; ++v) te[v][jt] = M::mma(av, dF[ks][v], te[v][jt]);
+                for (int v = 0; v < 5; ++v) te[v][jt] = M::mma(ae[s][jt][ks], dF[ks][v], te[v][jt]);
             }
         if (SPLIT && s == 1) {
             // side 1's differences from its own mean flux: + (F(m_0) - F(m_1)) n 2A ICc[cb1][j]
-            const T* ic = a.ICc + (size_t)cb1 * NP;
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const T d = (T)((ref0[v] - ref1[v]) * J);
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int j = 16 * jt + LY::jrow(r, grp);
-                        te[v][jt][r] += d * ic[min(j, NP - 1)];
-                    }
+                    for (int r = 0; r < 4; ++r) te[v][jt][r] += d * icv[jt][r];
             }
         }
         if (f >= 0) {
