@@ -110,6 +110,31 @@ def test_c5_tgv_p4_series():
         assert r[f"l2_rel_deviation_{prec}_vs_fp64"] < 1e-6, prec
 
 
+def test_c5_protocol_full_size():
+    """C5's protocol: the inviscid Taylor-Green vortex on 40^3 x 6 = 384,000 tets at P4 (Roe, CFL 0.3),
+    1,000 SSP-RK3 steps in FP64 and BF16x3 (FP32 as well): the integrated kinetic-energy and
+    enstrophy series every 100 steps within 1e-8 / 1e-7 (relative) of the FP64 series, mass and
+    total energy within 1e-12 (FP32) / 1e-9 (BF16x3), the final state within 1e-6 (relative L2)."""
+    print("[case] C5: TGV 40^3x6 P4 roe, 1000 steps, fp64/fp32/bf16x3 series", flush=True)
+    r = validate.run("tgv", n=40, order=4, steps=1000, every=100, jitter=0.0, precisions=("fp64", "fp32", "bf16x3"))
+    assert r["tets"] == 384000
+    S64 = np.array(r["fp64"]["series"])
+    assert S64.shape == (11, 7) and np.isfinite(S64).all()
+    # mass and total energy conserved; at t ~ 0.1 the kinetic energy has barely changed (it trades
+    # with internal energy in compressible flow: no sign is fixed)
+    assert abs(S64[-1, 3] / S64[0, 3] - 1) < 1e-12 and abs(S64[-1, 4] / S64[0, 4] - 1) < 1e-12
+    assert abs(S64[-1, 1] / S64[0, 1] - 1) < 1e-5
+    for prec in ("fp32", "bf16x3"):
+        dev = r[f"series_max_rel_deviation_{prec}"]
+        print(f"C5 {prec}: series deviation {dev}, state {r[f'l2_rel_deviation_{prec}_vs_fp64']:.2e}", flush=True)
+        assert dev["kinetic_energy"] < 1e-8 and dev["enstrophy"] < 1e-7, prec
+        # FP32 keeps the cell means' face sums in FP64 (face tiles); the BF16x3 split path sums the
+        # mean row of its FP32 face test in FP32
+        cons = 1e-12 if prec == "fp32" else 1e-9
+        assert dev["mass"] < cons and dev["total_energy"] < cons, prec
+        assert r[f"l2_rel_deviation_{prec}_vs_fp64"] < 1e-6, prec
+
+
 def test_c3_residual_full_size(solver, oracle):
     """C3 at full size: the 60^3 x 6 = 1,296,000-tet jittered periodic cube, P3 Roe; the FP64 residual
     within 1e-12 and the FP32-contraction residual within 2e-5 of the oracle's (all host threads)."""
