@@ -185,6 +185,55 @@ def test_c3_protocol_500_steps(solver):
     assert np.allclose(e32, e64, rtol=1e-3, atol=0)
 
 
+def test_c4_protocol_full_size():
+    """C4's protocol: the 256-problem draw (seed 256: n ~ U{6..24} hexes per axis, P1..P4, strength
+    U(1,5), centre U(3,7)), one ensemble per precision (order groups, HIP graph, autotuned), 100
+    SSP-RK3 steps each at its own stable dt: the FP32-contraction ensemble within 1e-6 (relative L2)
+    of the FP64 ensemble for every problem, and both with the same analytic vortex error."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    print("[case] C4: 256 problems, 100 steps, fp64 and fp32 ensembles", flush=True)
+    rng = np.random.default_rng(256)
+    P = 256
+    ns, orders = rng.integers(6, 25, P), rng.integers(1, 5, P)
+    strength, centre = rng.uniform(1, 5, P), rng.uniform(3, 7, (P, 3))
+    params = [[1.4, strength[p], centre[p, 0], centre[p, 1], 1.0, 1.0, 10.0, 10.0] for p in range(P)]
+
+    def make(p):
+        m = F.Mesh.box(int(ns[p]), ub=(10, 10, 10), periodic=7)
+        return m, m.project(int(orders[p]), "vortex", params[p])
+
+    with ThreadPoolExecutor(16) as ex:
+        built = list(ex.map(make, range(P)))
+    meshes = [b[0] for b in built]
+    h = np.array([3.0 * m.geometry()["h"].min() for m in meshes])
+    steps, out, dt = 100, {}, None
+    for prec in ("fp64", "fp32"):
+        ens = F.Ensemble(meshes, [int(o) for o in orders], flux="roe", precision=prec)
+        try:
+            for p in range(P):
+                ens.upload(p, built[p][1])
+            if dt is None:
+                dt = 0.3 * h / ((2 * orders + 1) * ens.max_wave_speeds())
+            ens.set_dt(dt)
+            ens.autotune(trial_steps=1)
+            for p in range(P):  # the autotuner's trials restore the state; start from U0 regardless
+                ens.upload(p, built[p][1])
+            ens.step(steps, graph=True)
+            out[prec] = [ens.download(p) for p in range(P)]
+        finally:
+            ens.close()
+        print(f"C4 {prec}: {steps} steps done", flush=True)
+    dev = np.array([np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()) for a, b in zip(out["fp32"], out["fp64"])])
+    print(f"C4: fp32 vs fp64 per-problem relative L2, max {dev.max():.2e}, median {np.median(dev):.2e}", flush=True)
+    assert all(np.isfinite(u).all() for u in out["fp32"])
+    assert dev.max() < 1e-6
+    for p in range(0, P, 32):
+        e64 = meshes[p].analytic_error(int(orders[p]), "vortex", params[p], steps * dt[p], out["fp64"][p])[[0, 1, 2, 4]]
+        e32 = meshes[p].analytic_error(int(orders[p]), "vortex", params[p], steps * dt[p], out["fp32"][p])[[0, 1, 2, 4]]
+        assert np.allclose(e32, e64, rtol=1e-3, atol=0), (p, e32, e64)
+
+
 def test_c4_subset_matches_oracle(oracle):
     """C4's draw (seed 256: n ~ U{6..24}, P1..P4, strength U(1,5), centre U(3,7)): the first 12
     problems as one ensemble (graph-replayed, autotuned), one SSP-RK3 step, each problem against the
