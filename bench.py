@@ -33,6 +33,9 @@ element kernel to the face pass: face pass F = 80QfN + 410Qf, element kernel F =
 per launch measured by rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a device copy)
 for this workload, read from profiles/hbm_traffic_<workload>.txt (null when absent).  `stage`
 gives the same counts for the whole stage, with the measured bytes beside the algorithmic ones.
+Peaks: FP64 78.6 TFLOP/s; the mixed paths price their contraction flops at the FP32 matrix peak
+(157.3), BF16x3's volume contractions at the bf16 matrix peak / 6 (six bf16 products per
+product), and the rest at FP64.
 
 Printed: ONE JSON line.  The CPU baseline (the oracle, a C restatement with the reference's loop
 structure, OpenMP) runs on this host: all threads on the full workload for at least one stage,
@@ -53,6 +56,7 @@ sys.path.insert(0, str(ROOT))
 PEAK_HBM = 8.0e12      # B/s
 PEAK_FP64 = 78.6e12    # flop/s (MI355X dense FP64, vector = matrix)
 PEAK_FP32 = 157.3e12   # flop/s (MI355X dense FP32 matrix)
+PEAK_BF16 = 2.5e15     # flop/s (MI355X dense bf16 matrix); BF16x3 spends 6 bf16 products per product
 
 METRIC = "DOF-updates/s per RK stage, P3 tet Euler: FP64 vs mixed; % HBM/MFMA roofline"
 
@@ -126,10 +130,11 @@ def roofline(ms, names, E, Q, Qf, N, precision, traffic):
         name = names.get(label[slot]) or slot
         if precision == "fp64":
             peak_f = PEAK_FP64
-        else:  # contractions at the FP32 matrix peak, the rest (pointwise, Riemann, update) at FP64
+        else:  # contractions at the FP32 matrix peak (BF16x3 volume: bf16 peak / 6), the rest at FP64
             Fc = {"volume": 40 * Q * N, "flux": (80 if tiles else 40) * Qf * N, "test": 40 * Qf * N,
                   "elem": 40 * Q * N + (0 if tiles else 40 * Qf * N)}[slot]
-            peak_f = F / (Fc / PEAK_FP32 + (F - Fc) / PEAK_FP64)
+            Fv = 40 * Q * N if (precision == "bf16x3" and slot in ("volume", "elem")) else 0
+            peak_f = F / (Fv / (PEAK_BF16 / 6) + (Fc - Fv) / PEAK_FP32 + (F - Fc) / PEAK_FP64)
         t_s = t * 1e-3
         tf, tb = F * E / peak_f, B * E / PEAK_HBM
         bound = "mfma" if tf >= tb else "hbm"
@@ -152,7 +157,8 @@ def roofline(ms, names, E, Q, Qf, N, precision, traffic):
     if precision == "fp64":
         peak_stage = PEAK_FP64
     else:
-        peak_stage = Fs / (c["contraction"] / PEAK_FP32 + (Fs - c["contraction"]) / PEAK_FP64)
+        Fv = 40 * Q * N if precision == "bf16x3" else 0
+        peak_stage = Fs / (Fv / (PEAK_BF16 / 6) + (c["contraction"] - Fv) / PEAK_FP32 + (Fs - c["contraction"]) / PEAK_FP64)
     mfma = k["bound"] == "mfma"
     return {
         "bound": k["bound"],
@@ -317,7 +323,9 @@ def run_single(args, w):
                                      "volume flux perturbation; FP64 Riemann central flux, pressures and positivity "
                                      "(FP32 upwind dissipation), face flux integrals, state, update and sums"),
         "mixed_bf16x3": mixed_block("bf16x3", "volume contractions as 6 bf16 matrix-core products of 3-part splits "
-                                              "(FP32 accumulation), face contractions FP32, FP64 state and fluxes"),
+                                              "(FP32 accumulation) of the FP32 path's differences from each "
+                                              "cell's mean; face traces and face test on the FP32 matrix cores; "
+                                              "FP64 Riemann central flux, pressures, state, update and sums"),
         "limited": {"limiter": "PositiveLimiter apply_1 + apply_2 after every stage (fp64)",
                     "ms_per_step": round(3 * ms_lim[0], 4),
                     "overhead_vs_fp64": round(ms_lim[0] / f64["ms"][0] - 1, 4)},

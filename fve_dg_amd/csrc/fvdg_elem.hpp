@@ -27,6 +27,7 @@
 // sum_q w_q grad phi_j added in FP64, and each face's FP64 reference flux enters as
 // sum_slots (+-Fref) sum_g w_g PhiC_c[g][j] (table IC).
 #pragma once
+#include "fvdg_bf16x3.hpp"
 #include "fvdg_kernels_fast.hpp"
 
 template <typename T>
@@ -58,6 +59,8 @@ struct ElemArgs {
     int* bad;
     const T* Cf;          // CF: per (face, side) test contributions [F][2][5][NP] of k_face_tile
     double phi0;          // CF, split: the constant value of phi_0 (MF in Fref: the faces' flux integrals)
+    const __bf16* VB3;    // BF3: volume operand tables of k_vol_bf16x3, 3 parts each
+    const __bf16* GB3;
 };
 
 template <int ORDER, typename T>
@@ -99,7 +102,11 @@ __device__ __forceinline__ void flux_delta_f32(const float pb[6], const A4 (&u)[
     }
 }
 
-template <int ORDER, typename T, bool FUSE, bool CF = false>
+// BF3 (T = float, CF): both volume contractions as the six bf16 part products of k_vol_bf16x3
+// (three-part splits of the tile's modes >= 1 and of the FP32 perturbation fluxes, FP32
+// accumulation); its accumulator layout is that of the FP32 16x16x4 form, so the face
+// contributions and the update are unchanged.
+template <int ORDER, typename T, bool FUSE, bool CF = false, bool BF3 = false>
 __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_mfma(ElemArgs<T> a) {
     using S = VolShape<ORDER>;
     using M = Mfma<T>;
@@ -190,7 +197,77 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
         for (int jt = 0; jt < JT; ++jt) racc[v][jt] = (acc_t){0, 0, 0, 0};
 
     // 3. volume term
-    if (do_cells) {
+    static_assert(!BF3 || (CF && !std::is_same<T, double>::value), "BF3: the FP32 face-tile form only");
+    if constexpr (BF3) {
+        using S3 = fvdg_bf3::Bf3Shape<ORDER>;
+        using fvdg_bf3::bf16x8;
+        constexpr int KS3 = S3::KS;
+        const float gm1 = (float)(a.gamma - 1.0);
+#pragma unroll 1
+        for (int t = 0; t < QT; ++t) {
+            // interpolation of modes >= 1: U_v[q][e] - c0_v = sum_k V[q][k] C_v[k][e]
+            acc_t uacc[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) uacc[v] = (acc_t){0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KS3; ++ks) {
+                bf16x8 av[3];
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+                    av[p] = *reinterpret_cast<const bf16x8*>(a.VB3 + (size_t)p * S3::VB + ((t * KS3 + ks) * 64 + l) * 8);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    bf16x8 bv[3];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int k = 32 * ks + 8 * grp + i, kc = min(k, N - 1);
+                        const double x = sU[col * STR + 5 * kc + v];
+                        fvdg_bf3::split3((k < N && k > 0) ? (float)x : 0.0f, bv, i);
+                    }
+                    uacc[v] = fvdg_bf3::mma6(av, bv, uacc[v]);
+                    asm volatile("" ::: "memory");  // one variable's LDS reads and splits at a time
+                }
+            }
+            // F(U_q) - F(c_0) at the lane's four points (FP32 perturbation form), times A^{-1} and -w |J|
+            float Bq[5][12];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = 16 * t + 4 * grp + r;
+                float dF[5][3];
+                flux_delta_f32(pb, uacc, r, gm1, dF);
+                const float jw = -(float)sW[q] * detf;  // zero weight on padding points
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        Bq[v][3 * r + d] = (dF[v][0] * Af[d] + dF[v][1] * Af[3 + d] + dF[v][2] * Af[6 + d]) * jw;
+            }
+            // test: K slices sl of (point, direction) pairs 8 sl + i of each lane group; the slice's
+            // table operands once, then one variable's split fluxes at a time (registers)
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                bf16x8 av[JT][3];
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p)
+                        av[jt][p] = *reinterpret_cast<const bf16x8*>(a.GB3 + (size_t)p * S3::GB +
+                                                                      (((t * 2 + sl) * JT + jt) * 64 + l) * 8);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    bf16x8 bv[3];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int pp = 8 * sl + i;
+                        fvdg_bf3::split3(pp < 12 ? Bq[v][pp] : 0.0f, bv, i);
+                    }
+#pragma unroll
+                    for (int jt = 0; jt < JT; ++jt) racc[v][jt] = fvdg_bf3::mma6(av[jt], bv, racc[v][jt]);
+                    __builtin_amdgcn_sched_barrier(0);  // one variable's splits at a time
+                }
+            }
+        }
+    } else if (do_cells) {
 #pragma unroll 1
         for (int t = 0; t < QT; ++t) {
             // this point tile's interpolation operands and the first point row's test operands

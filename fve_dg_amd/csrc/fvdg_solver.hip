@@ -277,7 +277,7 @@ bool use_face_tile(const fvdg_ctx* c) {
     if (c->face_tile == 0 || c->simple || c->owner_faces || c->lds_face_test || c->lds_face_flux || !c->elem_fused)
         return false;
     if (c->order < 2 || c->order > 4 || c->tab.Qf != 24 || c->n_face_tiles == 0) return false;
-    if (c->precision == FVDG_PREC_FP32) return true;
+    if (c->precision == FVDG_PREC_FP32 || c->precision == FVDG_PREC_BF16X3) return true;
     return c->precision == FVDG_PREC_FP64 && c->face_tile == 2;
 }
 
@@ -296,7 +296,7 @@ int precision_buffers(fvdg_ctx* c) {
     // autotuner switches the path on and off per order group)
     const size_t ncf = (size_t)c->F * 2 * 5 * ((c->tab.N + 3) / 4 * 4);
     const int ft = c->face_tile;
-    if (c->precision == FVDG_PREC_FP32 && c->face_tile == 0) c->face_tile = 1;
+    if (c->precision != FVDG_PREC_FP64 && c->face_tile == 0) c->face_tile = 1;
     const bool tiles = use_face_tile(c);
     c->face_tile = ft;
     if (tiles) {
@@ -690,7 +690,16 @@ void elem_path_t(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up,
         }
         Span se(ms_elem, c->stream);
         const dim3 ge(nblocks(c->E, 16));
-        if (tiles) {
+        if (tiles && c->precision == FVDG_PREC_BF16X3) {
+            if constexpr (TileShape<P>::SUPPORTED && !std::is_same<T, double>::value) {
+                a.VB3 = reinterpret_cast<const __bf16*>(c->tVB3.p);
+                a.GB3 = reinterpret_cast<const __bf16*>(c->tGB3.p);
+                if (up)
+                    hipLaunchKernelGGL((k_elem_mfma<P, T, true, true, true>), ge, dim3(64), 0, c->stream, a);
+                else
+                    hipLaunchKernelGGL((k_elem_mfma<P, T, false, true, true>), ge, dim3(64), 0, c->stream, a);
+            }
+        } else if (tiles) {
             if (up)
                 hipLaunchKernelGGL((k_elem_mfma<P, T, true, true>), ge, dim3(64), 0, c->stream, a);
             else
@@ -712,11 +721,12 @@ void elem_path_p(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up,
         elem_path_t<P, double>(c, U, parts, up, c->tPhiC.p, c->Fh.p, c->tVA.p, c->tGA.p, c->tPhiB.p, ms_flux,
                                ms_elem);
 }
-// the fused element path serves P0-P4 in FP64 and FP32 (BF16x3, P5, the alternative face kernels,
-// the plain kernels and FVDG_ELEM=split take the volume kernel + face-test kernel path)
-bool use_elem(const fvdg_ctx* c) {
+// the fused element path serves P0-P4 in FP64 and FP32, and BF16x3 where the face tiles serve it
+// (P2-P4, the whole residual); P5, BF16x3 otherwise, the alternative face kernels, the plain
+// kernels and FVDG_ELEM=split take the volume kernel + face-test kernel path
+bool use_elem(const fvdg_ctx* c, int parts) {
     return c->elem_fused && !c->simple && !c->owner_faces && !c->lds_face_test && c->order <= 4 &&
-           c->precision != FVDG_PREC_BF16X3;
+           (c->precision != FVDG_PREC_BF16X3 || (parts == FVDG_PART_ALL && use_face_tile(c)));
 }
 void elem_path(fvdg_ctx* c, const double* U, int parts, const StageUpdate* up, double* ms_flux, double* ms_elem) {
     switch (c->order) {
@@ -761,7 +771,7 @@ int launch_residual(fvdg_ctx* c, const double* U, int parts, double* ms_vol, dou
             hipLaunchKernelGGL(k_face_flux, dim3(nblocks(c->F * T.Qf, bs)), dim3(bs), 0, c->stream, a);
         hipLaunchKernelGGL(k_face_test, dim3(nblocks(c->E * T.N, bs)), dim3(bs), 0, c->stream, a);
         ss.stop();
-    } else if (use_elem(c)) {
+    } else if (use_elem(c, parts)) {
         Span ss(ms_surf, c->stream);
         elem_path(c, U, parts, nullptr, nullptr, nullptr);
         ss.stop();
@@ -839,7 +849,7 @@ int ssprk3_stage(fvdg_ctx* c, double dt, int s, double* ms) {
                                dst[s], c->cgeo.p, c->tMhat.p, c->E, c->tab.N, A[s], B[s], dt,
                                c->edt.n ? c->edt.p : nullptr, c->bad.p);
             su.stop();
-        } else if (use_elem(c) && !fused_extrema) {
+        } else if (use_elem(c, FVDG_PART_ALL) && !fused_extrema) {
             const StageUpdate up{c->U.p, dst[s], A[s], B[s], dt};
             elem_path(c, src[s], FVDG_PART_ALL, &up, ms ? &ms[2] : nullptr, ms ? &ms[3] : nullptr);
         } else {
@@ -871,9 +881,10 @@ std::string stage_kernel_names(const fvdg_ctx* c) {
     const bool dpp = P >= 2 && P <= 4 && !c->lds_face_flux;
     const std::string flux = dpp ? "k_face_flux_dpp<" + ps + "," + T + "," + std::to_string(c->flux_threads) + ">"
                                  : "k_face_flux_lds<" + ps + "," + T + ">";
-    if (use_elem(c) && !fused_extrema && use_face_tile(c))
-        return "volume=;flux=k_face_tile<" + ps + "," + T + ">;test=k_elem_mfma<" + ps + "," + T + ",true,true>";
-    if (use_elem(c) && !fused_extrema) return "volume=;flux=" + flux + ";test=k_elem_mfma<" + ps + "," + T + ",true>";
+    if (use_elem(c, FVDG_PART_ALL) && !fused_extrema && use_face_tile(c))
+        return "volume=;flux=k_face_tile<" + ps + "," + T + ">;test=k_elem_mfma<" + ps + "," + T + ",true,true" +
+               (c->precision == FVDG_PREC_BF16X3 ? ",true>" : ">");
+    if (use_elem(c, FVDG_PART_ALL) && !fused_extrema) return "volume=;flux=" + flux + ";test=k_elem_mfma<" + ps + "," + T + ",true>";
     std::string vol = c->precision == FVDG_PREC_BF16X3 ? "k_vol_bf16x3<" + ps + ">"
                                                        : "k_vol_mfma<" + ps + "," + T + (fused_extrema ? ",true>" : ">");
     if (c->owner_faces && !mx) return "volume=" + vol + ";flux=k_surf_lds<" + ps + ",true>;test=";

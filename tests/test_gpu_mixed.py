@@ -77,6 +77,31 @@ def test_precision_switch_keeps_the_state(solver):
     assert 0 < rel_err(r32, r64) < 2e-5
 
 
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_bf16x3_stage_on_face_tiles(solver, oracle, order):
+    """BF16x3 at P2-P4: the stage runs the FP32 face tiles and the element kernel with the volume
+    contractions on the bf16 matrix cores (k_elem_mfma<P,float,true,true,true>); its residual
+    differs from the FP32 path's in the last bits and stays within 2e-5 of the oracle."""
+    print(f"[case] bf16x3 face-tile stage P{order}", flush=True)
+    mesh = F.Mesh.box(4, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=9)
+    U = mesh.project(order, "vortex", VORTEX)
+    U *= 1 + 1e-2 * np.random.default_rng(order).standard_normal(U.shape)
+    R = {}
+    for p in ("fp32", "bf16x3"):
+        solver.set_scheme(order, flux="roe", precision=p).set_mesh(mesh)
+        k = solver.stage_kernels()
+        assert k["volume"] == "" and k["flux"] == f"k_face_tile<{order},float>", k
+        assert k["test"] == f"k_elem_mfma<{order},float,true,true" + (",true>" if p == "bf16x3" else ">"), k
+        solver.upload(U)
+        R[p] = solver.residual()
+    assert not np.array_equal(R["bf16x3"], R["fp32"])
+    Ro = oracle.residual(mesh.export(), order, U, flux="roe")
+    for p in ("fp32", "bf16x3"):
+        err = rel_err(R[p], Ro)
+        print(f"{p} residual vs oracle: {err:.3e}", flush=True)
+        assert err < 2e-5, p
+
+
 @pytest.mark.parametrize("order", [1, 3, 5])
 def test_bf16x3_volume_is_its_own_kernel(solver, order):
     """The BF16x3 volume term differs from the FP32 one in the last bits (another contraction
