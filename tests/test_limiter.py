@@ -299,11 +299,12 @@ def test_reference_form_limiter_near_vacuum(oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("order", [2, 3])
-def test_gpu_stage_limiter_mixed_precision(solver, oracle, order):
-    """The stage limiter after FP32-contraction stages (face tiles at P2-P3): one limited SSP-RK3
-    step within 2e-5 of the oracle's limited FP64 step (the FP32 residual's bar)."""
-    print(f"[case] stage limiter fp32 P{order}", flush=True)
+def test_gpu_stage_limiter_mixed_precision(solver, oracle, order, prec):
+    """The stage limiter after FP32-contraction or BF16x3 stages (face tiles at P2-P3): one limited
+    SSP-RK3 step within 2e-5 of the oracle's limited FP64 step (the mixed residual's bar)."""
+    print(f"[case] stage limiter {prec} P{order}", flush=True)
     mesh = MESHES["periodic"]()
     m = mesh.export()
     U0 = mesh.project(order, "vortex", VORTEX[:2] + [1.0, 1.0, 0.5, 0.5, 2.0, 2.0])
@@ -321,7 +322,8 @@ def test_gpu_stage_limiter_mixed_precision(solver, oracle, order):
     U2 = lim(0.75 * U0 + 0.25 * (U1 + dt * L(U1)), U1)
     exp = lim(U0 / 3 + 2.0 / 3 * (U2 + dt * L(U2)), U2)
     assert rel_err(exp, U0 + dt * L(U0)) > 1e-6  # the limiter changed something
-    solver.set_scheme(order, flux="roe", precision="fp32").set_mesh(mesh)
+    solver.set_scheme(order, flux="roe", precision=prec).set_mesh(mesh)
+    assert solver.stage_kernels()["flux"].startswith("k_face_tile")
     solver.upload(U0)
     solver.set_stage_limiter(3)
     try:
