@@ -34,8 +34,8 @@ per launch measured by rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a d
 for this workload, read from profiles/hbm_traffic_<workload>.txt (null when absent).  `stage`
 gives the same counts for the whole stage, with the measured bytes beside the algorithmic ones.
 Peaks: FP64 78.6 TFLOP/s; the mixed paths price their contraction flops at the FP32 matrix peak
-(157.3), BF16x3's volume contractions at the bf16 matrix peak / 6 (six bf16 products per
-product), and the rest at FP64.
+(157.3), BF16x3's contractions (the volume and, on its face tiles, the faces) at the bf16 matrix
+peak / 6 (six bf16 products per product), and the rest at FP64.
 
 Printed: ONE JSON line.  The CPU baseline (the oracle, a C restatement with the reference's loop
 structure, OpenMP) runs on this host: all threads on the full workload for at least one stage,
@@ -117,6 +117,7 @@ def roofline(ms, names, E, Q, Qf, N, precision, traffic):
     fused = not names.get("volume")
     # face tiles: the face pass also runs the face test, the element kernel the volume and update
     tiles = (names.get("flux") or "").startswith("k_face_tile")
+    bf3_faces = (names.get("flux") or "").startswith("k_face_tile_bf3")
     if tiles:
         c["flux"] = (c["flux"][0] + 40 * Qf * N, c["flux"][1])
         c["elem"] = (c["elem"][0] - 40 * Qf * N, c["elem"][1])
@@ -134,6 +135,8 @@ def roofline(ms, names, E, Q, Qf, N, precision, traffic):
             Fc = {"volume": 40 * Q * N, "flux": (80 if tiles else 40) * Qf * N, "test": 40 * Qf * N,
                   "elem": 40 * Q * N + (0 if tiles else 40 * Qf * N)}[slot]
             Fv = 40 * Q * N if (precision == "bf16x3" and slot in ("volume", "elem")) else 0
+            if bf3_faces and slot == "flux":
+                Fv = Fc  # the face tiles' traces and test as BF16x3 products
             peak_f = F / (Fv / (PEAK_BF16 / 6) + (Fc - Fv) / PEAK_FP32 + (F - Fc) / PEAK_FP64)
         t_s = t * 1e-3
         tf, tb = F * E / peak_f, B * E / PEAK_HBM
@@ -157,7 +160,7 @@ def roofline(ms, names, E, Q, Qf, N, precision, traffic):
     if precision == "fp64":
         peak_stage = PEAK_FP64
     else:
-        Fv = 40 * Q * N if precision == "bf16x3" else 0
+        Fv = (40 * Q * N if precision == "bf16x3" else 0) + (80 * Qf * N if bf3_faces else 0)
         peak_stage = Fs / (Fv / (PEAK_BF16 / 6) + (c["contraction"] - Fv) / PEAK_FP32 + (Fs - c["contraction"]) / PEAK_FP64)
     mfma = k["bound"] == "mfma"
     return {
@@ -324,7 +327,7 @@ def run_single(args, w):
                                      "(FP32 upwind dissipation), face flux integrals, state, update and sums"),
         "mixed_bf16x3": mixed_block("bf16x3", "volume contractions as 6 bf16 matrix-core products of 3-part splits "
                                               "(FP32 accumulation) of the FP32 path's differences from each "
-                                              "cell's mean; face traces and face test on the FP32 matrix cores; "
+                                              "cell's mean, face traces and face test likewise (face tiles); "
                                               "FP64 Riemann central flux, pressures, state, update and sums"),
         "limited": {"limiter": "PositiveLimiter apply_1 + apply_2 after every stage (fp64)",
                     "ms_per_step": round(3 * ms_lim[0], 4),

@@ -98,6 +98,9 @@ struct fvdg_ctx {
     DevBuf<double> tAt, tAe;              // FP64 operand layouts
     DevBuf<float> tAt32, tAe32;           // FP32 operand layouts
     DevBuf<float> tICc32;                 // [24][NP] sum_g w_g Phi_cb[g][j] (FP32 side-1 reference term)
+    DevBuf<uint16_t> tAt3, tAe3;          // BF16x3 face-tile operand tables (k_face_tile_bf3)
+    bool bf3_faces = true;                // BF16x3: the face contractions as bf16 part products too
+                                          // (FVDG_BF16X3_FACES=0: FP32 face contractions)
     DevBuf<double> Cf;                    // FP64 contributions [F][2][5][NP]
     DevBuf<float> Cf32;                   // FP32 contributions
     int n_cu = 256;

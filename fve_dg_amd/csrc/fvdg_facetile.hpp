@@ -31,6 +31,7 @@
 // polynomials exactly); the cell-mean row (j = 0, phi_0 constant) takes the FP64 face sums
 // +-phi_0 MF instead of its FP32 entries, so that the cell means stay conserved in FP64.
 #pragma once
+#include "fvdg_bf16x3.hpp"
 #include "fvdg_kernels_fast.hpp"
 
 // Lane layouts of the face tile.  Lane l = (col = l % 16: face, grp = l / 16) holds six face
@@ -83,10 +84,17 @@ struct FaceTileArgs {
     const T* ICc;     // [24][NP] sum_g w_g Phi_c[g][j] (T = float)
     T* Cf;            // [F][2][5][NP]
     double* MF;       // [F][5] sum_g Fhat_g w_g 2A (T = float)
+    const __bf16* At3;  // BF3 faces: [3 parts][24][2][KS3][64][8] trace operands
+    const __bf16* Ae3;  // BF3 faces: [3 parts][24][JT][64][8] test operands
 };
 
-template <int ORDER, typename T>
-__global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
+// BF3 (T = float): both face contractions as the six bf16 part products of three-part splits
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation; the accumulator layout is that of the FP32
+// 16x16x4 form, so the Riemann solves and the stores are unchanged): the traces with K = the
+// lane group's 8 modes of each 32-mode step, the test with K = the lane group's six point slots
+// (and two zeros), each lane's six differences split once for both sides.
+template <int ORDER, typename T, bool BF3>
+__device__ __forceinline__ void face_tile_body(const FaceTileArgs<T>& a) {
     using SH = TileShape<ORDER>;
     using M = Mfma<T>;
     using LY = TileLayout<T>;
@@ -124,13 +132,15 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
     for (int s = 0; s < 6; ++s) wg[s] = a.wf[LY::point(s, grp)];
     // the trace operands of both sides (boundary tiles: side 1 unused)
     T at[2][2][KS];
+    if constexpr (!BF3) {
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb)
+            for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-                at[s][mb][ks] = a.At[((size_t)(s ? (two ? cb1 : cb0) : cb0) * 2 * KS + mb * KS + ks) * 64 + l];
+                for (int ks = 0; ks < KS; ++ks)
+                    at[s][mb][ks] = a.At[((size_t)(s ? (two ? cb1 : cb0) : cb0) * 2 * KS + mb * KS + ks) * 64 + l];
+    }
     __syncthreads();
 
     // 1. coefficient blocks into LDS: row grp loads blocks 4 i + grp, lane col entries 16 r + col;
@@ -178,10 +188,39 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
         for (int v = 0; v < 5; ++v)
 #pragma unroll
             for (int mb = 0; mb < 2; ++mb) tr[s][v][mb] = (acc_t){0, 0, 0, 0};
+    constexpr int KS3 = (N + 31) / 32;
+    static_assert(!BF3 || SPLIT, "BF3 face contractions: the FP32 form only");
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         if (s == 1 && !two) break;
         const T* B = sC + (s * 16 + col) * STR;
+        if constexpr (BF3) {
+            const int cbs = s ? (two ? cb1 : cb0) : cb0;
+#pragma unroll
+            for (int ks = 0; ks < KS3; ++ks) {
+                fvdg_bf3::bf16x8 a3[2][3];
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p)
+                        a3[mb][p] = *reinterpret_cast<const fvdg_bf3::bf16x8*>(
+                            a.At3 + (size_t)p * 24 * 2 * KS3 * 512 + ((((size_t)cbs * 2 + mb) * KS3 + ks) * 64 + l) * 8);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    fvdg_bf3::bf16x8 bv[3];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int k = 32 * ks + 8 * grp + i, kc = min(k, N - 1);
+                        const T bx = B[5 * kc + v];
+                        fvdg_bf3::split3(k < N ? bx : 0.0f, bv, i);
+                    }
+                    tr[s][v][0] = fvdg_bf3::mma6(a3[0], bv, tr[s][v][0]);
+                    tr[s][v][1] = fvdg_bf3::mma6(a3[1], bv, tr[s][v][1]);
+                    asm volatile("" ::: "memory");  // one variable's LDS reads and splits at a time
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int k = 4 * ks + grp, kc = min(k, N - 1);  // rows k >= N: zero operand (branch-free)
@@ -249,7 +288,7 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
     }
     // the test operands of both sides and side 1's mean-row correction, in flight during the solves
     // (FP32 at P2, P3; otherwise each side's operands are loaded at its products: registers)
-    constexpr bool PRE = SPLIT && JT <= 2;
+    constexpr bool PRE = SPLIT && JT <= 2 && !BF3;
     const int cbs1 = two ? cb1 : cb0;
     T ae[2][JT][6], icv[JT][4];
     auto load_ae = [&](int s) __attribute__((always_inline)) {
@@ -293,22 +332,43 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
     }
 
     // 4./5. test products per side and the stores
+    // BF3: the lane's six differences of each variable split once (K entries 8 grp + slot)
+    fvdg_bf3::bf16x8 d3[BF3 ? 5 : 1][3];
+    if constexpr (BF3)
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) fvdg_bf3::split3(i < 6 ? (float)dF[i < 6 ? i : 0][v] : 0.0f, d3[v], i);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         if (s == 1 && !two) break;
-        if (!PRE) load_ae(s);
         acc_t te[5][JT];
 #pragma unroll
         for (int v = 0; v < 5; ++v)
 #pragma unroll
             for (int jt = 0; jt < JT; ++jt) te[v][jt] = (acc_t){0, 0, 0, 0};
-#pragma unroll
-        for (int ks = 0; ks < 6; ++ks)
+        if constexpr (BF3) {
+            const int cbs = s ? cbs1 : cb0;
 #pragma unroll
             for (int jt = 0; jt < JT; ++jt) {
+                fvdg_bf3::bf16x8 a3[3];
 #pragma unroll
-                for (int v = 0; v < 5; ++v) te[v][jt] = M::mma(ae[s][jt][ks], dF[ks][v], te[v][jt]);
+                for (int p = 0; p < 3; ++p)
+                    a3[p] = *reinterpret_cast<const fvdg_bf3::bf16x8*>(
+                        a.Ae3 + (size_t)p * 24 * JT * 512 + (((size_t)cbs * JT + jt) * 64 + l) * 8);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) te[v][jt] = fvdg_bf3::mma6(a3, d3[v], te[v][jt]);
             }
+        } else {
+            if (!PRE) load_ae(s);
+#pragma unroll
+            for (int ks = 0; ks < 6; ++ks)
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt) {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) te[v][jt] = M::mma(ae[s][jt][ks], dF[ks][v], te[v][jt]);
+                }
+        }
         if (SPLIT && s == 1) {
             // side 1's differences from its own mean flux: + (F(m_0) - F(m_1)) n 2A ICc[cb1][j]
 #pragma unroll
@@ -347,4 +407,15 @@ __global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
                 }
         }
     }
+}
+
+template <int ORDER, typename T>
+__global__ __launch_bounds__(64, 2) void k_face_tile(FaceTileArgs<T> a) {
+    face_tile_body<ORDER, T, false>(a);
+}
+
+// the face tile of the BF16x3 precision: both face contractions as BF16x3 products
+template <int ORDER>
+__global__ __launch_bounds__(64, 2) void k_face_tile_bf3(FaceTileArgs<float> a) {
+    face_tile_body<ORDER, float, true>(a);
 }

@@ -197,6 +197,43 @@ int upload_face_tile_tables(fvdg_ctx* c, DevBuf<T>& At, DevBuf<T>& Ae, DevBuf<T>
     return FVDG_OK;
 }
 
+// BF16x3 operand tables of k_face_tile_bf3 (three bf16 parts, part stride = one table):
+//   At3[cb][mb][ks][lane][i] = Phi_cb[row_point(mb, lane % 16)][32 ks + 8 (lane / 16) + i]   (traces)
+//   Ae3[cb][jt][lane][i]     = Phi_cb[point(i, lane / 16)][16 jt + lane % 16], i < 6; 0 for i = 6, 7 (test)
+int upload_face_tile_bf3_tables(fvdg_ctx* c) {
+    const SchemeTables& S = c->tab;
+    if (S.Qf != 24 || S.N < 10 || S.N > 35) {
+        c->tAt3.release();
+        c->tAe3.release();
+        return FVDG_OK;
+    }
+    using LY = TileLayout<float>;
+    const int N = S.N, KS3 = (N + 31) / 32, JT = (N + 15) / 16;
+    auto phi = [&](int cb, int g, int k) { return S.Phi[((size_t)cb * S.Qf + g) * N + k]; };
+    const size_t nat = (size_t)24 * 2 * KS3 * 512, nae = (size_t)24 * JT * 512;
+    std::vector<uint16_t> at(3 * nat, 0), ae(3 * nae, 0);
+    for (int cb = 0; cb < 24; ++cb) {
+        for (int mb = 0; mb < 2; ++mb)
+            for (int ks = 0; ks < KS3; ++ks)
+                for (int ln = 0; ln < 64; ++ln)
+                    for (int i = 0; i < 8; ++i) {
+                        const int g = LY::row_point(mb, ln % 16), k = 32 * ks + 8 * (ln / 16) + i;
+                        const double x = (g >= 0 && k < N) ? phi(cb, g, k) : 0.0;
+                        split_bf16x3(x, at, nat, ((((size_t)cb * 2 + mb) * KS3 + ks) * 64 + ln) * 8 + i);
+                    }
+        for (int jt = 0; jt < JT; ++jt)
+            for (int ln = 0; ln < 64; ++ln)
+                for (int i = 0; i < 8; ++i) {
+                    const int j = 16 * jt + ln % 16;
+                    const double x = (i < 6 && j < N) ? phi(cb, LY::point(i, ln / 16), j) : 0.0;
+                    split_bf16x3(x, ae, nae, (((size_t)cb * JT + jt) * 64 + ln) * 8 + i);
+                }
+    }
+    HIP_TRY(c->tAt3.upload(at, c->stream));
+    HIP_TRY(c->tAe3.upload(ae, c->stream));
+    return FVDG_OK;
+}
+
 int upload_tables(fvdg_ctx* c) {
     const SchemeTables& T = c->tab;
     HIP_TRY(c->tV.upload(T.V, c->stream));
@@ -265,6 +302,8 @@ int upload_tables(fvdg_ctx* c) {
         if (rc) return rc;
         rc = upload_face_tile_tables<float>(c, c->tAt32, c->tAe32, &c->tICc32);
         if (rc) return rc;
+        rc = upload_face_tile_bf3_tables(c);
+        if (rc) return rc;
     }
     HIP_TRY(c->tCanon.upload(canon, c->stream));
     HIP_TRY(c->tPerm.upload(perm, c->stream));
@@ -279,6 +318,11 @@ bool use_face_tile(const fvdg_ctx* c) {
     if (c->order < 2 || c->order > 4 || c->tab.Qf != 24 || c->n_face_tiles == 0) return false;
     if (c->precision == FVDG_PREC_FP32 || c->precision == FVDG_PREC_BF16X3) return true;
     return c->precision == FVDG_PREC_FP64 && c->face_tile == 2;
+}
+
+// BF16x3 (unless FVDG_BF16X3_FACES=0): the face tiles' contractions as bf16 part products too
+bool bf3_face_tiles(const fvdg_ctx* c) {
+    return c->precision == FVDG_PREC_BF16X3 && c->bf3_faces && c->tAt3.n > 0 && c->tAe3.n > 0;
 }
 
 // Scratch buffers of the contraction precision (face fluxes, and for the mixed path the
@@ -630,6 +674,15 @@ void face_tile_pass(fvdg_ctx* c, const double* U, const T* At, const T* Ae, T* C
         FaceTileArgs<T> f{c->n_face_tiles, c->gamma, c->flux, U, reinterpret_cast<const int4*>(c->trec.p),
                           c->tclass.p, c->fgeo.p, c->UD.p, c->tWf.p, At, Ae, ICc, Cf, c->Fref.p};
         Span s1(ms_flux, c->stream);
+        if constexpr (!std::is_same<T, double>::value) {
+            if (bf3_face_tiles(c)) {
+                f.At3 = reinterpret_cast<const __bf16*>(c->tAt3.p);
+                f.Ae3 = reinterpret_cast<const __bf16*>(c->tAe3.p);
+                hipLaunchKernelGGL((k_face_tile_bf3<P>), dim3((unsigned)c->n_face_tiles), dim3(64), 0, c->stream, f);
+                s1.stop();
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_face_tile<P, T>), dim3((unsigned)c->n_face_tiles), dim3(64), 0, c->stream, f);
         s1.stop();
     }
@@ -882,8 +935,8 @@ std::string stage_kernel_names(const fvdg_ctx* c) {
     const std::string flux = dpp ? "k_face_flux_dpp<" + ps + "," + T + "," + std::to_string(c->flux_threads) + ">"
                                  : "k_face_flux_lds<" + ps + "," + T + ">";
     if (use_elem(c, FVDG_PART_ALL) && !fused_extrema && use_face_tile(c))
-        return "volume=;flux=k_face_tile<" + ps + "," + T + ">;test=k_elem_mfma<" + ps + "," + T + ",true,true" +
-               (c->precision == FVDG_PREC_BF16X3 ? ",true>" : ">");
+        return "volume=;flux=" + (bf3_face_tiles(c) ? "k_face_tile_bf3<" + ps + ">" : "k_face_tile<" + ps + "," + T + ">") +
+               ";test=k_elem_mfma<" + ps + "," + T + ",true,true" + (c->precision == FVDG_PREC_BF16X3 ? ",true>" : ">");
     if (use_elem(c, FVDG_PART_ALL) && !fused_extrema) return "volume=;flux=" + flux + ";test=k_elem_mfma<" + ps + "," + T + ",true>";
     std::string vol = c->precision == FVDG_PREC_BF16X3 ? "k_vol_bf16x3<" + ps + ">"
                                                        : "k_vol_mfma<" + ps + "," + T + (fused_extrema ? ",true>" : ">");
@@ -970,6 +1023,7 @@ int fvdg_create(fvdg_ctx** out, int device) {
     if (const char* gm = std::getenv("FVDG_GRID_MULT")) c->grid_mult = std::max(1, std::atoi(gm));
     if (const char* to = std::getenv("FVDG_TILE_ORDER"))
         c->tile_order = std::strcmp(to, "class") == 0 ? 0 : std::strcmp(to, "space") == 0 ? 1 : 2;
+    if (const char* bf = std::getenv("FVDG_BF16X3_FACES")) c->bf3_faces = std::atoi(bf) != 0;
     if (const char* ft = std::getenv("FVDG_FACE_TILE"))
         c->face_tile = std::strcmp(ft, "off") == 0 ? 0 : std::strcmp(ft, "all") == 0 ? 2 : 1;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

@@ -8,6 +8,8 @@ oracle residual and differs from it (the mixed kernels really ran); a trajectory
 1e-6 relative L2 of the FP64 trajectory; its error against the analytic vortex equals the FP64
 path's to 1e-3 relative; the cell-mean integrals stay conserved.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -79,9 +81,9 @@ def test_precision_switch_keeps_the_state(solver):
 
 @pytest.mark.parametrize("order", [2, 3, 4])
 def test_bf16x3_stage_on_face_tiles(solver, oracle, order):
-    """BF16x3 at P2-P4: the stage runs the FP32 face tiles and the element kernel with the volume
-    contractions on the bf16 matrix cores (k_elem_mfma<P,float,true,true,true>); its residual
-    differs from the FP32 path's in the last bits and stays within 2e-5 of the oracle."""
+    """BF16x3 at P2-P4: the stage runs the face tiles and the element kernel with every contraction
+    on the bf16 matrix cores (k_face_tile_bf3<P>, k_elem_mfma<P,float,true,true,true>); its
+    residual differs from the FP32 path's in the last bits and stays within 2e-5 of the oracle."""
     print(f"[case] bf16x3 face-tile stage P{order}", flush=True)
     mesh = F.Mesh.box(4, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=9)
     U = mesh.project(order, "vortex", VORTEX)
@@ -90,7 +92,8 @@ def test_bf16x3_stage_on_face_tiles(solver, oracle, order):
     for p in ("fp32", "bf16x3"):
         solver.set_scheme(order, flux="roe", precision=p).set_mesh(mesh)
         k = solver.stage_kernels()
-        assert k["volume"] == "" and k["flux"] == f"k_face_tile<{order},float>", k
+        flux = f"k_face_tile_bf3<{order}>" if p == "bf16x3" else f"k_face_tile<{order},float>"
+        assert k["volume"] == "" and k["flux"] == flux, k
         assert k["test"] == f"k_elem_mfma<{order},float,true,true" + (",true>" if p == "bf16x3" else ">"), k
         solver.upload(U)
         R[p] = solver.residual()
@@ -187,3 +190,52 @@ def test_mixed_face_paths_agree(oracle, order, n, prec, monkeypatch):
         assert rel_err(R, Ro) < 2e-5, path
         assert rel_err(R, out["default"][0]) < 3e-6, path
         assert rel_err(U2, out["default"][1]) < 1e-6, path
+
+
+@pytest.fixture(scope="module")
+def fp32_faces_solver():
+    old = os.environ.get("FVDG_BF16X3_FACES")
+    os.environ["FVDG_BF16X3_FACES"] = "0"
+    try:
+        s = F.Solver(0)
+    finally:
+        if old is None:
+            del os.environ["FVDG_BF16X3_FACES"]
+        else:
+            os.environ["FVDG_BF16X3_FACES"] = old
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_bf16x3_face_contractions(fp32_faces_solver, solver, oracle, order):
+    """BF16x3's face tiles run the traces and the face test as BF16x3 products (k_face_tile_bf3);
+    FVDG_BF16X3_FACES=0 keeps those two in FP32.  Both residuals within 2e-5 of the oracle, not
+    equal; 20 SSP-RK3 steps of each within 1e-6 (relative L2) of FP64."""
+    print(f"[case] bf16x3 face contractions P{order}", flush=True)
+    mesh = F.Mesh.box(4, ub=(10, 10, 10), periodic=7, jitter=0.1, seed=12)
+    U = mesh.project(order, "vortex", VORTEX)
+    U *= 1 + 1e-2 * np.random.default_rng(order).standard_normal(U.shape)
+    Ro = oracle.residual(mesh.export(), order, U, flux="roe")
+    R = {}
+    for name, s, flux in (("bf3", solver, f"k_face_tile_bf3<{order}>"),
+                          ("fp32", fp32_faces_solver, f"k_face_tile<{order},float>")):
+        s.set_scheme(order, flux="roe", precision="bf16x3").set_mesh(mesh)
+        assert s.stage_kernels()["flux"] == flux
+        s.upload(U)
+        R[name] = s.residual()
+        err = rel_err(R[name], Ro)
+        print(f"bf16x3 ({name} faces) residual vs oracle: {err:.3e}", flush=True)
+        assert err < 2e-5
+    assert not np.array_equal(R["bf3"], R["fp32"])
+    U0 = mesh.project(order, "vortex", VORTEX)
+    out = {}
+    for name, s, prec in (("fp64", solver, "fp64"), ("bf3", solver, "bf16x3"), ("fp32", fp32_faces_solver, "bf16x3")):
+        s.set_scheme(order, flux="roe", precision=prec).set_mesh(mesh)
+        s.upload(U0)
+        s.ssprk3(0.01, 20)
+        out[name] = s.download()
+    for name in ("bf3", "fp32"):
+        dev = float(np.sqrt(((out[name] - out["fp64"]) ** 2).sum() / (out["fp64"] ** 2).sum()))
+        print(f"bf16x3 ({name} faces) vs fp64 after 20 steps: {dev:.2e}", flush=True)
+        assert dev < 1e-6
