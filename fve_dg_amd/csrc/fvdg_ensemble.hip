@@ -11,7 +11,7 @@
 // becomes its branches) and replayed per step.
 //
 // The autotuner times, per order group on the ensemble's own data, the stage's kernel
-// choices: for FP32 groups at P2-P4 the face tiles (16 faces of one vertex-combination class per
+// choices: for FP32 and BF16x3 groups at P2-P4 the face tiles (16 faces of one vertex-combination class per
 // wavefront, both face contractions on the matrix cores); the fused element kernel or the volume +
 // face-test kernels, with the row-broadcast face-flux kernel in 128-, 256- or 512-thread blocks
 // (16 faces per 256 threads) or the LDS-staged one; keeping the fastest; the state is restored
@@ -325,16 +325,19 @@ int fvdg_ensemble_autotune(fvdg_ensemble* e, int trial_steps, int* choices) {
         HIP_TRY(hipMemcpyAsync(save.p, c->U.p, nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         // candidates: (fused element kernel, face pass: -1 = face tiles, 0 = LDS-staged face flux, else
         // the row-broadcast face flux's block size)
+        // (BF16x3: its fused element kernel runs only on its face tiles)
         const bool dpp_ok = c->order >= 2 && c->order <= 4;
-        const bool elem_ok = c->order <= 4 && c->precision != FVDG_PREC_BF16X3;
-        const bool tile_ok = elem_ok && c->precision == FVDG_PREC_FP32 && c->n_face_tiles > 0 && c->tab.Qf == 24 &&
-                             c->order >= 2;
+        const bool bf3 = c->precision == FVDG_PREC_BF16X3;
+        const bool tile_ok = c->order >= 2 && c->order <= 4 && c->precision != FVDG_PREC_FP64 &&
+                             c->n_face_tiles > 0 && c->tab.Qf == 24;
+        const bool elem_ok = c->order <= 4 && (!bf3 || tile_ok);
         double best = 1e300;
         int best_elem = c->elem_fused, best_flux = c->lds_face_flux ? 0 : c->flux_threads;
         const int tile0 = c->face_tile;
         for (int elem = elem_ok ? 1 : 0; elem >= 0; --elem)
             for (int flux : {-1, 256, 128, 512, 0}) {
                 if (flux == -1 && (!tile_ok || elem == 0)) continue;
+                if (bf3 && elem == 1 && flux != -1) continue;
                 if (flux > 0 && !dpp_ok) continue;
                 if (flux == 0 && dpp_ok && elem == 0) continue;  // the split path with the LDS flux: not competitive
                 c->elem_fused = elem != 0;

@@ -99,14 +99,19 @@ def test_autotune_keeps_state_and_results(oracle):
     ens.close()
 
 
-def test_mixed_precision_ensemble_tracks_fp64():
-    """FP32 and BF16x3 contractions in a graph-replayed ensemble stay within 1e-6 (relative L2) of
-    the FP64 ensemble after 10 steps, problem by problem."""
-    print("[case] ensemble fp32 / bf16x3 vs fp64", flush=True)
+@pytest.mark.parametrize("tuned", [False, True])
+def test_mixed_precision_ensemble_tracks_fp64(tuned):
+    """FP32 and BF16x3 contractions in a graph-replayed ensemble (as built, or after the autotuner
+    picked each group's kernels: BF16x3 groups at P2-P4 may take their face tiles) stay within 1e-6
+    (relative L2) of the FP64 ensemble after 10 steps, problem by problem."""
+    print(f"[case] ensemble fp32 / bf16x3 vs fp64, autotuned {tuned}", flush=True)
     out = {}
     for prec in ("fp64", "fp32", "bf16x3"):
         _, ens, _ = build(precision=prec)
         ens.set_dt(np.full(len(SPECS), 0.01))
+        if tuned:
+            choice = ens.autotune(trial_steps=1)
+            print(f"{prec}: {choice}", flush=True)
         ens.step(10, graph=True)
         out[prec] = [ens.download(p) for p in range(len(SPECS))]
         ens.close()
