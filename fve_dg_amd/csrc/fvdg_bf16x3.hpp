@@ -1,8 +1,8 @@
 // gfx950 volume kernel of the BF16x3 precision (FVDG_PREC_BF16X3): the volume term of the FP32
 // path (k_vol_mfma<ORDER, float>: mode 0 and the mean flux F(c0) kept in FP64, the variation in
 // FP32) with both contractions on the bf16 matrix cores.  Every FP32 operand x is split into three
-// bf16 parts, x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1), 24
-// significant bits together), and a product a.b is formed from the six part products
+// bf16 parts, x = x0 + x1 + x2 (24 significant bits together: the host rounds the tables' parts,
+// the kernels truncate the operands' parts, split3x8), and a product a.b is formed from the six part products
 // a_i.b_j with i + j <= 2 (the dropped terms are below FP32 rounding), accumulated in FP32 by
 // v_mfma_f32_16x16x32_bf16.
 //
@@ -56,6 +56,30 @@ __device__ __forceinline__ void split3(float x, bf16x8 (&b)[3], int i) {
     b[0][i] = h;
     b[1][i] = m;
     b[2][i] = (__bf16)(r - (float)m);
+}
+
+// the three-part splits of eight values into b[0..2] (element i of each part vector): truncating
+// splits, x0 = x with its low 16 bits cleared, x1 = the same of x - x0, x2 = the top 16 bits of
+// x - x0 - x1 (both subtractions exact; x0 + x1 + x2 within 2^-24 |x| of x), two values packed per
+// 32-bit word by one byte permute (the top halves of both words)
+__device__ __forceinline__ void split3x8(const float (&x)[8], bf16x8 (&b)[3]) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 w0, w1, w2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t a = __float_as_uint(x[2 * k]), c = __float_as_uint(x[2 * k + 1]);
+        const float ra = x[2 * k] - __uint_as_float(a & 0xffff0000u);
+        const float rc = x[2 * k + 1] - __uint_as_float(c & 0xffff0000u);
+        const uint32_t ua = __float_as_uint(ra), uc = __float_as_uint(rc);
+        const float ta = ra - __uint_as_float(ua & 0xffff0000u);
+        const float tc = rc - __uint_as_float(uc & 0xffff0000u);
+        w0[k] = __builtin_amdgcn_perm(c, a, 0x07060302u);
+        w1[k] = __builtin_amdgcn_perm(uc, ua, 0x07060302u);
+        w2[k] = __builtin_amdgcn_perm(__float_as_uint(tc), __float_as_uint(ta), 0x07060302u);
+    }
+    b[0] = __builtin_bit_cast(bf16x8, w0);
+    b[1] = __builtin_bit_cast(bf16x8, w1);
+    b[2] = __builtin_bit_cast(bf16x8, w2);
 }
 
 // sum over the six part products with i + j <= 2
@@ -132,12 +156,13 @@ __global__ __launch_bounds__(64, Bf3Shape<ORDER>::MIN_WAVES) void k_vol_bf16x3(B
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 bf16x8 bv[3];
+                float xs[8];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const int k = 32 * ks + 8 * grp + i;
-                    const float x = k < N ? sC[col * STR + 5 * k + v] : 0.0f;
-                    split3(x, bv, i);
+                    xs[i] = k < N ? sC[col * STR + 5 * k + v] : 0.0f;
                 }
+                split3x8(xs, bv);
                 uacc[v] = mma6(av, bv, uacc[v]);
             }
         }
@@ -172,12 +197,12 @@ __global__ __launch_bounds__(64, Bf3Shape<ORDER>::MIN_WAVES) void k_vol_bf16x3(B
         for (int s = 0; s < 2; ++s) {
             bf16x8 bv[5][3];
 #pragma unroll
-            for (int v = 0; v < 5; ++v)
+            for (int v = 0; v < 5; ++v) {
+                float xs[8];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int p = 8 * s + i;
-                    split3(p < 12 ? B[v][p] : 0.0f, bv[v], i);
-                }
+                for (int i = 0; i < 8; ++i) xs[i] = 8 * s + i < 12 ? B[v][8 * s + i] : 0.0f;
+                split3x8(xs, bv[v]);
+            }
 #pragma unroll
             for (int jt = 0; jt < JT; ++jt) {
                 bf16x8 av[3];

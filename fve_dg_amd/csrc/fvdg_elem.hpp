@@ -218,12 +218,14 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     bf16x8 bv[3];
+                    float xs[8];
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int k = 32 * ks + 8 * grp + i, kc = min(k, N - 1);
                         const double x = sU[col * STR + 5 * kc + v];
-                        fvdg_bf3::split3((k < N && k > 0) ? (float)x : 0.0f, bv, i);
+                        xs[i] = (k < N && k > 0) ? (float)x : 0.0f;
                     }
+                    fvdg_bf3::split3x8(xs, bv);
                     uacc[v] = fvdg_bf3::mma6(av, bv, uacc[v]);
                     asm volatile("" ::: "memory");  // one variable's LDS reads and splits at a time
                 }
@@ -256,11 +258,10 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     bf16x8 bv[3];
+                    float xs[8];
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const int pp = 8 * sl + i;
-                        fvdg_bf3::split3(pp < 12 ? Bq[v][pp] : 0.0f, bv, i);
-                    }
+                    for (int i = 0; i < 8; ++i) xs[i] = 8 * sl + i < 12 ? Bq[v][8 * sl + i] : 0.0f;
+                    fvdg_bf3::split3x8(xs, bv);
 #pragma unroll
                     for (int jt = 0; jt < JT; ++jt) racc[v][jt] = fvdg_bf3::mma6(av[jt], bv, racc[v][jt]);
                     __builtin_amdgcn_sched_barrier(0);  // one variable's splits at a time

@@ -208,12 +208,14 @@ __device__ __forceinline__ void face_tile_body(const FaceTileArgs<T>& a) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     fvdg_bf3::bf16x8 bv[3];
+                    float xs[8];
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int k = 32 * ks + 8 * grp + i, kc = min(k, N - 1);
                         const T bx = B[5 * kc + v];
-                        fvdg_bf3::split3(k < N ? bx : 0.0f, bv, i);
+                        xs[i] = k < N ? bx : 0.0f;
                     }
+                    fvdg_bf3::split3x8(xs, bv);
                     tr[s][v][0] = fvdg_bf3::mma6(a3[0], bv, tr[s][v][0]);
                     tr[s][v][1] = fvdg_bf3::mma6(a3[1], bv, tr[s][v][1]);
                     asm volatile("" ::: "memory");  // one variable's LDS reads and splits at a time
@@ -336,9 +338,12 @@ __device__ __forceinline__ void face_tile_body(const FaceTileArgs<T>& a) {
     fvdg_bf3::bf16x8 d3[BF3 ? 5 : 1][3];
     if constexpr (BF3)
 #pragma unroll
-        for (int v = 0; v < 5; ++v)
+        for (int v = 0; v < 5; ++v) {
+            float xs[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) fvdg_bf3::split3(i < 6 ? (float)dF[i < 6 ? i : 0][v] : 0.0f, d3[v], i);
+            for (int i = 0; i < 8; ++i) xs[i] = i < 6 ? (float)dF[i < 6 ? i : 0][v] : 0.0f;
+            fvdg_bf3::split3x8(xs, d3[v]);
+        }
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         if (s == 1 && !two) break;
