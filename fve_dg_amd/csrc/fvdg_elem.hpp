@@ -230,10 +230,12 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                     asm volatile("" ::: "memory");  // one variable's LDS reads and splits at a time
                 }
             }
-            // F(U_q) - F(c_0) at the lane's four points (FP32 perturbation form), times A^{-1} and -w |J|
+            // F(U_q) - F(c_0) at the lane's four points (FP32 perturbation form), times A^{-1} and -w |J|;
+            // K slice 0 takes pairs 0..7 (points 0, 1 and two directions of point 2), slice 1 pairs
+            // 8..11: points 0-2 are formed before slice 0's products and point 3 after them (registers)
             float Bq[5][12];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            auto point = [&](auto RC) __attribute__((always_inline)) {
+                constexpr int r = decltype(RC)::value;
                 const int q = 16 * t + 4 * grp + r;
                 float dF[5][3];
                 flux_delta_f32(pb, uacc, r, gm1, dF);
@@ -243,11 +245,11 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
 #pragma unroll
                     for (int d = 0; d < 3; ++d)
                         Bq[v][3 * r + d] = (dF[v][0] * Af[d] + dF[v][1] * Af[3 + d] + dF[v][2] * Af[6 + d]) * jw;
-            }
+            };
             // test: K slices sl of (point, direction) pairs 8 sl + i of each lane group; the slice's
             // table operands once, then one variable's split fluxes at a time (registers)
-#pragma unroll
-            for (int sl = 0; sl < 2; ++sl) {
+            auto slice = [&](auto SL) __attribute__((always_inline)) {
+                constexpr int sl = decltype(SL)::value;
                 bf16x8 av[JT][3];
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt)
@@ -266,7 +268,13 @@ __global__ __launch_bounds__(64, (ElemShape<ORDER, T>::MIN_WAVES)) void k_elem_m
                     for (int jt = 0; jt < JT; ++jt) racc[v][jt] = fvdg_bf3::mma6(av[jt], bv, racc[v][jt]);
                     __builtin_amdgcn_sched_barrier(0);  // one variable's splits at a time
                 }
-            }
+            };
+            point(std::integral_constant<int, 0>{});
+            point(std::integral_constant<int, 1>{});
+            point(std::integral_constant<int, 2>{});
+            slice(std::integral_constant<int, 0>{});
+            point(std::integral_constant<int, 3>{});
+            slice(std::integral_constant<int, 1>{});
         }
     } else if (do_cells) {
 #pragma unroll 1

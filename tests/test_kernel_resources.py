@@ -29,13 +29,13 @@ BUDGET = {
     "k_elem_mfma<3,float,true,true,false>": (12, 0),
     "k_elem_mfma<3,float,true,true,true>": (8, 0),
     "k_face_tile_bf3<3>": (8, 0),
-    # P4 (C5): 23 KB element and face-tile tiles hold six wavefronts per compute unit; the FP64 and
-    # BF16x3 element kernels spill at their 256-register cap
+    # P4 (C5): 23 KB element and face-tile tiles hold six wavefronts per compute unit; the FP64
+    # element kernel spills at its 256-register cap
     "k_face_flux_dpp<4,double,256>": (8, 0),  # 56 KB of tables per block
     "k_elem_mfma<4,double,true,false,false>": (6, 252),
     "k_face_tile<4,float>": (6, 0),
     "k_elem_mfma<4,float,true,true,false>": (6, 0),
-    "k_elem_mfma<4,float,true,true,true>": (6, 128),
+    "k_elem_mfma<4,float,true,true,true>": (6, 0),
     "k_face_tile_bf3<4>": (6, 0),
     # P1, P2 (C1, the ensemble's groups)
     "k_face_flux_lds<1,double>": (12, 0),
