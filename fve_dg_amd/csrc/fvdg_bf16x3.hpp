@@ -48,16 +48,6 @@ struct Bf3Args {
     const double* Gs;  // N*3: sum_q w_q d phi_j / d xi_d
 };
 
-// element i of the three part vectors b[0..2]
-__device__ __forceinline__ void split3(float x, bf16x8 (&b)[3], int i) {
-    const __bf16 h = (__bf16)x;
-    const float r = x - (float)h;
-    const __bf16 m = (__bf16)r;
-    b[0][i] = h;
-    b[1][i] = m;
-    b[2][i] = (__bf16)(r - (float)m);
-}
-
 // the three-part splits of eight values into b[0..2] (element i of each part vector): truncating
 // splits, x0 = x with its low 16 bits cleared, x1 = the same of x - x0, x2 = the top 16 bits of
 // x - x0 - x1 (both subtractions exact; x0 + x1 + x2 within 2^-24 |x| of x), two values packed per
